@@ -1,0 +1,221 @@
+/*
+ * mcgpu_energy.h -- C ABI of the MI355X-native MCGPU energy hot path.
+ *
+ * One shared library (mcgpu_amd/libmcgpu.so) exports every function below.
+ * Plain pointers and sizes only; host buffers are owned by the caller, device
+ * buffers by the library.  Every call returns 0 (MCGPU_OK) or a negative
+ * error code; mcgpu_last_error() gives the message of the calling thread's
+ * last failure.  A handle is not thread-safe: one host thread per handle.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * MCGPU source tree):
+ *   SimBox SoA + GPUCopy residency     src/Metropolis/SimBox.h:18-612,
+ *                                      src/Metropolis/GPUCopy.cpp:97-222
+ *   calcMolecularEnergyContribution    src/Metropolis/SimulationStep.h:54,
+ *                                      src/Metropolis/BruteForceStep.cpp:15-79
+ *   calcSystemEnergy                   src/Metropolis/SimulationStep.h:93,
+ *                                      src/Metropolis/SimulationStep.cpp:40-71
+ *   changeMolecule / rollback          src/Metropolis/SimulationStep.h:74,83,
+ *                                      src/Metropolis/SimulationStep.cpp:520-566,731-749
+ *   NLC / Verlet rebuild               src/Metropolis/NLCStep.cpp:125-253,
+ *                                      src/Metropolis/VerletStep.cpp:119-207
+ *   NPT resize + volume acceptance     src/Metropolis/SimulationStep.cpp:476-518,
+ *                                      src/Metropolis/Ensemble.cpp:9-28
+ *   MC step loop (batched, on device)  src/Metropolis/Simulation.cpp:219-264
+ */
+#ifndef MCGPU_ENERGY_H
+#define MCGPU_ENERGY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MCGPU_OK 0
+#define MCGPU_ERR_INVALID (-1)  /* bad argument or descriptor */
+#define MCGPU_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define MCGPU_ERR_NOMEM (-3)    /* device or host allocation failed / budget */
+#define MCGPU_ERR_RANGE (-4)    /* index out of range */
+#define MCGPU_ERR_STATE (-5)    /* call not valid in the current state */
+#define MCGPU_ERR_NODEVICE (-6) /* no HIP device visible */
+
+/* ---- pair arithmetic ---- */
+#define MCGPU_FP64 0  /* every term and sum in FP64 */
+#define MCGPU_MIXED 1 /* pair LJ/Coulomb in FP32, displacements, sums,
+                         coordinates and Metropolis in FP64 */
+
+/* ---- neighbour structure used by energy calls ---- */
+#define MCGPU_NEIGHBOR_BRUTE 0  /* scan every molecule (BruteForceStep) */
+#define MCGPU_NEIGHBOR_CELL 1   /* cell list, width >= rc + primary spread */
+#define MCGPU_NEIGHBOR_VERLET 2 /* Verlet list with skin */
+
+/* ---- ensembles ---- */
+#define MCGPU_NVT 0
+#define MCGPU_NPT 1
+
+/*
+ * Box descriptor: the SimBox fields the energy path reads
+ * (SimBox.h: atomCoordinates, atomData, moleculeData rows MOL_START,
+ * MOL_LEN, MOL_PIDX_START, MOL_PIDX_COUNT, primaryIndexes, size, cutoff,
+ * kT, maxTranslate, maxRotate).  All arrays are host memory, read during
+ * mcgpu_box_create only.  Sites with sigma < 0 or epsilon < 0 are dummy
+ * sites: they move with their molecule but never interact
+ * (BruteForceStep.cpp:64-65).
+ */
+typedef struct mcgpu_box_desc {
+  int32_t n_molecules;
+  int32_t n_sites;
+  int32_t n_primary; /* length of primary_index */
+  int32_t reserved;
+  double box[3];        /* edge lengths, Angstrom */
+  double cutoff;        /* molecule cutoff on primary sites, Angstrom */
+  double kT;            /* kcal/mol (T * 0.00198717) */
+  double max_translate; /* Angstrom */
+  double max_rotate;    /* degrees */
+  const double* x;      /* [n_sites] */
+  const double* y;
+  const double* z;
+  const double* sigma;   /* [n_sites], Angstrom */
+  const double* epsilon; /* [n_sites], kcal/mol */
+  const double* charge;  /* [n_sites], e */
+  const int32_t* mol_start;      /* [n_molecules] first site */
+  const int32_t* mol_len;        /* [n_molecules] site count */
+  const int32_t* mol_pidx_start; /* [n_molecules] into primary_index */
+  const int32_t* mol_pidx_count; /* [n_molecules] */
+  const int32_t* primary_index;  /* [n_primary] absolute site index */
+} mcgpu_box_desc;
+
+/* Largest molecule the kernels accept (sites per molecule). */
+#define MCGPU_MAX_MOL_SITES 32
+
+typedef struct mcgpu_box mcgpu_box;
+
+const char* mcgpu_last_error(void);
+int mcgpu_device_count(int* count);
+/* Blocks until all work the library queued on the device is done. */
+int mcgpu_device_synchronize(void);
+
+/* Creates the device-resident box (replaces SimBoxBuilder::build +
+ * GPUCopy::copyIn).  precision: MCGPU_FP64 or MCGPU_MIXED. */
+int mcgpu_box_create(const mcgpu_box_desc* desc, int precision,
+                     mcgpu_box** out);
+int mcgpu_box_destroy(mcgpu_box* box);
+int mcgpu_box_set_precision(mcgpu_box* box, int precision);
+/* Selects the neighbour structure and (re)builds it.  kind: MCGPU_NEIGHBOR_*.
+ * skin is used for MCGPU_NEIGHBOR_VERLET only (Angstrom). */
+int mcgpu_neighbor_rebuild(mcgpu_box* box, int kind, double skin);
+
+/* Whole-box coordinate transfer (GPUCopy::copyIn / copyOut). */
+int mcgpu_upload_coords(mcgpu_box* box, const double* x, const double* y,
+                        const double* z);
+int mcgpu_download_coords(mcgpu_box* box, double* x, double* y, double* z);
+/* Overwrites one molecule's sites: xyz = x block, y block, z block, each
+ * mol_len doubles (the shape of SimBox::rollBackCoordinates). */
+int mcgpu_set_molecule_coords(mcgpu_box* box, int32_t mol, const double* xyz);
+int mcgpu_get_molecule_coords(mcgpu_box* box, int32_t mol, double* xyz);
+
+/* E_mol(start) = sum over j in [start_mol, N), j != mol, molecules in range,
+ * of all real site pairs -- exactly calcMolecularEnergyContribution
+ * (BruteForceStep.cpp:15-46).  Synchronous. */
+int mcgpu_molecule_energy(mcgpu_box* box, int32_t mol, int32_t start_mol,
+                          double* energy);
+
+/* Sum_i E_i(i): the full-system intermolecular energy
+ * (SimulationStep.cpp:40-71 without the LRC).  Synchronous. */
+int mcgpu_system_energy(mcgpu_box* box, double* energy);
+/* Number of real site pairs summed by the last mcgpu_system_energy call. */
+int mcgpu_system_pair_count(mcgpu_box* box, int64_t* pairs);
+
+/* One trial move, fused: evaluates E_mol(0) at the current (old) and at the
+ * proposed coordinates in one pass, applies the Metropolis test of
+ * SimulationStep.cpp:794-803 with the caller's uniform draw u (used only when
+ * dE >= 0), and commits the proposed coordinates on acceptance.
+ * new_xyz: x block, y block, z block (mol_len each).  Synchronous. */
+typedef struct mcgpu_move_result {
+  double old_energy;
+  double new_energy;
+  double delta;   /* new_energy - old_energy */
+  int32_t accepted;
+  int32_t reserved;
+} mcgpu_move_result;
+int mcgpu_trial_move(mcgpu_box* box, int32_t mol, const double* new_xyz,
+                     double u, mcgpu_move_result* result);
+
+/* NPT: rescales the box by factor (molecules translated rigidly, primary
+ * site position multiplied by factor; replaces SimCalcs::resizeBox) and
+ * rebuilds the neighbour structure. */
+int mcgpu_volume_scale(mcgpu_box* box, double factor);
+int mcgpu_box_size(mcgpu_box* box, double* edges3);
+
+/* Device timing of the last kernel launched by a synchronous box call, in
+ * milliseconds (HIP events on the box's stream; enable first). */
+int mcgpu_box_enable_timing(mcgpu_box* box, int on);
+int mcgpu_last_kernel_ms(mcgpu_box* box, double* ms);
+
+/* ------------------------------------------------------------------ */
+/* Batched boxes: many independent boxes advanced by one persistent launch.
+ * Each box carries its own device RNG (a glibc TYPE_3 stream seeded like
+ * srand(seed), MathLibrary.cpp:27-33), its own Metropolis test and energy.
+ * A sweep slot = one trial move for every active box. */
+
+typedef struct mcgpu_run_params {
+  uint32_t seed;          /* MC seed (srand) */
+  int32_t precision;      /* MCGPU_FP64 / MCGPU_MIXED */
+  int32_t ensemble;       /* MCGPU_NVT / MCGPU_NPT */
+  int32_t volume_interval;/* NPT: a volume move every this many moves */
+  int64_t steps;          /* trial moves to run for this box */
+  double pressure;        /* NPT, kcal/mol/A^3 (1 atm = 1.4584e-5) */
+  double max_volume_delta;/* NPT, relative: dV = U(-1,1) * this * V */
+  double energy_offset;   /* added to the start energy (e.g. the LRC) */
+  int32_t neighbor;       /* MCGPU_NEIGHBOR_BRUTE or MCGPU_NEIGHBOR_CELL */
+  int32_t record_trace;   /* nonzero: keep per-move (mol, dE, accepted) */
+} mcgpu_run_params;
+
+typedef struct mcgpu_box_stats {
+  int64_t steps_done;
+  int64_t accepted;
+  int64_t volume_moves;
+  int64_t volume_accepted;
+  double energy;  /* running total energy (start energy + accepted dE) */
+  double start_energy;
+  double box[3];  /* current edges (change under NPT) */
+  int32_t done;
+  int32_t fault;  /* nonzero: box stopped (see mcgpu_last_error on add) */
+} mcgpu_box_stats;
+
+typedef struct mcgpu_batch mcgpu_batch;
+
+/* hbm_budget_bytes: admission limit for all resident boxes (0 = no limit). */
+int mcgpu_batch_create(int32_t max_boxes, size_t hbm_budget_bytes,
+                       mcgpu_batch** out);
+int mcgpu_batch_destroy(mcgpu_batch* b);
+/* Device bytes a box would take (for admission decisions). */
+int mcgpu_box_footprint(const mcgpu_box_desc* desc,
+                        const mcgpu_run_params* params, size_t* bytes);
+/* Adds a box; computes its start energy on the device. Returns
+ * MCGPU_ERR_NOMEM if the budget or slot count is exhausted. */
+int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* desc,
+                    const mcgpu_run_params* params, int32_t* slot);
+int mcgpu_batch_remove(mcgpu_batch* b, int32_t slot);
+/* Advances every active box by up to `sweeps` moves in one persistent
+ * launch (captured and replayed as a HIP graph when the box set is
+ * unchanged).  Asynchronous unless sync != 0. */
+int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync);
+int mcgpu_batch_stats(mcgpu_batch* b, int32_t slot, mcgpu_box_stats* out);
+int mcgpu_batch_download(mcgpu_batch* b, int32_t slot, double* x, double* y,
+                         double* z);
+/* Copies the recorded trace (if record_trace was set): up to cap moves. */
+int mcgpu_batch_trace(mcgpu_batch* b, int32_t slot, int32_t* mol,
+                      double* delta, int8_t* accepted, int64_t cap,
+                      int64_t* n);
+/* Average device duration (ms) of the batch kernel over the last
+ * mcgpu_batch_run calls since the previous query, and their count. */
+int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+/*
+ * mcgpu_host.h -- host-side C ABI around the energy path: synthetic
+ * workload generators, the .state format, the plain (one launch per move)
+ * MC driver and the serving loop.  Exported by mcgpu_amd/libmcgpu.so.
+ *
+ * Reference interfaces:
+ *   .state read/write        src/Metropolis/Utilities/StateScanner.cpp:18-195,
+ *                            235-281, 391-597; FileUtilities.cpp:119-283
+ *   MC driver loop           src/Metropolis/Simulation.cpp:65-264
+ *   RNG                      src/Metropolis/Utilities/MathLibrary.cpp:27-33
+ *   LRC                      src/Metropolis/SerialSim/SerialCalcs.cpp:37-108
+ */
+#ifndef MCGPU_HOST_H
+#define MCGPU_HOST_H
+
+#include "mcgpu_energy.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A host-resident box (owned arrays), convertible to an mcgpu_box_desc. */
+typedef struct mcgpu_hostbox mcgpu_hostbox;
+
+#define MCGPU_WL_WATER 0    /* rigid 3-site TIP3P water */
+#define MCGPU_WL_BICYCLIC 1 /* rigid 16-site planar bicyclic (9 heavy + 7 H) */
+#define MCGPU_WL_MIXTURE 2  /* water + 6-site methanol */
+
+typedef struct mcgpu_workload_params {
+  int32_t kind;        /* MCGPU_WL_* */
+  int32_t n_molecules; /* total molecules */
+  int32_t n_second;    /* MIXTURE: how many are methanol */
+  int32_t lattice_jitter_permille; /* lattice jitter, 1/1000 of spacing */
+  double density;      /* g/cm^3 (ignored when edge > 0) */
+  double edge;         /* cubic box edge in A, or 0 to derive from density */
+  double temperature;  /* K */
+  double cutoff;       /* A; clamped to edge/2 */
+  double max_translate;/* A */
+  double max_rotate;   /* degrees */
+  uint64_t seed;       /* generator seed (geometry, parameters, placement) */
+  uint32_t mc_seed;    /* MC seed recorded in the box (srand) */
+  int32_t reserved;
+} mcgpu_workload_params;
+
+int mcgpu_workload_generate(const mcgpu_workload_params* p,
+                            mcgpu_hostbox** out);
+int mcgpu_hostbox_free(mcgpu_hostbox* h);
+/* Descriptor pointing into h's arrays (valid until h is freed or changed). */
+int mcgpu_hostbox_desc(mcgpu_hostbox* h, mcgpu_box_desc* out);
+int mcgpu_hostbox_info(mcgpu_hostbox* h, double* temperature,
+                       uint32_t* mc_seed);
+/* Writable views of the coordinate arrays (for loading final states). */
+int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y, double** z);
+
+/* .state files, full precision (%.17g) so a reload is bit exact. */
+int mcgpu_state_write(mcgpu_hostbox* h, const char* path, int64_t step);
+int mcgpu_state_read(const char* path, mcgpu_hostbox** out, int64_t* step);
+/* .config + z-matrix + OPLS parameter inputs (ConfigScanner.cpp,
+ * ZmatrixScanner.cpp, OplsScanner.cpp, FileUtilities.cpp:285-642). */
+int mcgpu_config_read(const char* path, mcgpu_hostbox** out, int64_t* steps);
+
+/* The reference's LRC (SerialCalcs.cpp:37-108) and the K-species form. */
+int mcgpu_lrc_reference(const mcgpu_box_desc* d, double* e);
+int mcgpu_lrc_species(const mcgpu_box_desc* d, double* e);
+
+/* ---- plain driver: host RNG + one fused trial-move launch per move ---- */
+typedef struct mcgpu_plain_result {
+  double start_energy; /* pairs + LRC */
+  double lrc;
+  double final_energy;
+  int64_t accepted;
+  int64_t rejected;
+  double loop_seconds;    /* wall time of the step loop */
+  double startup_seconds; /* wall time of the start-up energy */
+} mcgpu_plain_result;
+
+/* Runs `steps` NVT moves of box h (mutated to the final state) with the
+ * reference's draw order and Metropolis rule.  Optional trace arrays of
+ * length steps (NULL to skip).  neighbor: MCGPU_NEIGHBOR_*. */
+int mcgpu_plain_run(mcgpu_hostbox* h, int32_t precision, int32_t neighbor,
+                    int64_t steps, int32_t* trace_mol, double* trace_de,
+                    int8_t* trace_acc, mcgpu_plain_result* out);
+
+/* The same loop through the reference-shaped strategy calls (two
+ * molecule-energy calls, changeMolecule, rollback per move), i.e. what a
+ * drop-in GPU SimulationStep does. */
+int mcgpu_strategy_run(mcgpu_hostbox* h, int64_t steps, int32_t* trace_mol,
+                       double* trace_de, int8_t* trace_acc,
+                       mcgpu_plain_result* out);
+
+/* ---- serving loop ---- */
+typedef struct mcgpu_server_config {
+  int32_t max_boxes;      /* resident slots */
+  int32_t sweeps_per_tick;/* moves per box per persistent launch */
+  size_t hbm_budget_bytes;/* admission budget */
+} mcgpu_server_config;
+
+typedef struct mcgpu_job_result {
+  int64_t job_id;
+  double arrival_s;
+  double admit_s;
+  double finish_s;
+  double final_energy;
+  double start_energy;
+  int64_t steps;
+  int64_t accepted;
+  int32_t status; /* 0 ok, <0 error code */
+  int32_t reserved;
+} mcgpu_job_result;
+
+typedef struct mcgpu_server mcgpu_server;
+int mcgpu_server_create(const mcgpu_server_config* cfg, mcgpu_server** out);
+int mcgpu_server_destroy(mcgpu_server* s);
+/* Queues a job (the server copies the box).  arrival_s: when it becomes
+ * visible, on the server clock (seconds since mcgpu_server_start). */
+int mcgpu_server_submit(mcgpu_server* s, mcgpu_hostbox* box,
+                        const mcgpu_run_params* params, double arrival_s,
+                        int64_t* job_id);
+/* Starts the clock and runs the continuous-batching loop until every
+ * submitted job has finished. */
+int mcgpu_server_run(mcgpu_server* s);
+int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out, int64_t cap,
+                         int64_t* n);
+typedef struct mcgpu_server_stats {
+  double wall_s;
+  int64_t total_moves;
+  int64_t ticks;
+  double kernel_ms_total;
+  int64_t peak_resident;
+  size_t peak_bytes;
+} mcgpu_server_stats;
+int mcgpu_server_get_stats(mcgpu_server* s, mcgpu_server_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,900 @@
+// Batched boxes: hundreds of independent boxes advanced by one persistent
+// launch.  One work-group owns one box for the whole launch and runs the
+// reference's step loop (Simulation.cpp:219-264) on the device: its own
+// glibc-TYPE_3 RNG stream (MathLibrary.cpp:27-33), the rigid trial move
+// (SimulationStep.cpp:520-566), old+new energy in one pass, the Metropolis
+// test (SimulationStep.cpp:794-803), commit or nothing (rollback is free:
+// proposals never touch the box).  No grid-wide synchronisation: boxes are
+// independent, so a launch is as long as its slowest box.
+//
+// Box memory is carved from one HBM arena (admission by footprint).  NPT
+// volume moves run between launches with the multi-work-group full-energy
+// kernel (see batch_volume_move).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "box_internal.h"
+#include "eval.h"
+#include "mcgpu_host.h"
+
+namespace mcgpu {
+
+// Per-box control/state, written by the kernel, mirrored to the host after
+// every launch.
+struct BoxCtl {
+  int64_t steps_done;
+  int64_t accepted;
+  int64_t steps_target;
+  int64_t next_stop;  // the kernel stops this box when steps_done reaches it
+  double energy;
+  int32_t status;  // kRunning / kDone / kVolumePending
+  int32_t pad;
+};
+constexpr int32_t kRunning = 0, kDone = 1, kVolumePending = 2;
+
+struct BatchBox {
+  BoxView bv;
+  Rng31* rng;
+  int32_t* tr_mol;
+  double* tr_de;
+  int8_t* tr_acc;
+  double max_t;
+  double max_r;
+  int32_t precision;
+  int32_t record;
+};
+
+// ------------------------------------------------------------- kernel
+
+// Proposed coordinates of the rigid move, built in LDS from the current
+// ones: every site except the pivot is rotated about the pivot by the X, Y,
+// Z Euler angles, then all sites are translated, then the molecule is
+// wrapped by its first primary site (SimulationStep.cpp:387-422, 545-565,
+// 710-729).  FMA contraction is off so the doubles match the host path.
+template <int NT>
+__device__ void propose_device(EvalShared<NT>& sh, const BoxView& bv, int vertex,
+                               const double* d, const double* cs,
+                               const double* sn) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int len = sh.len;
+  const double px = sh.ox[vertex], py = sh.oy[vertex], pz = sh.oz[vertex];
+  for (int k = t; k < len; k += NT) {
+    double X = sh.ox[k], Y = sh.oy[k], Z = sh.oz[k];
+    if (k != vertex) {
+      X += -px;
+      Y += -py;
+      Z += -pz;
+      double u = Y, v = Z;  // rotateX
+      Y = u * cs[0] + v * sn[0];
+      Z = v * cs[0] - u * sn[0];
+      u = Z;
+      v = X;  // rotateY
+      Z = u * cs[1] + v * sn[1];
+      X = v * cs[1] - u * sn[1];
+      u = X;
+      v = Y;  // rotateZ
+      X = u * cs[2] + v * sn[2];
+      Y = v * cs[2] - u * sn[2];
+      X += px;
+      Y += py;
+      Z += pz;
+    }
+    X += d[0];
+    Y += d[1];
+    Z += d[2];
+    sh.nx[k] = X;
+    sh.ny[k] = Y;
+    sh.nz[k] = Z;
+  }
+  __syncthreads();
+  const int p0 = sh.pofs[0];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  {
+    const double qx = sh.nx[p0], qy = sh.ny[p0], qz = sh.nz[p0];
+    sx = qx < 0 ? bv.L[0] : (qx > bv.L[0] ? -bv.L[0] : 0.0);
+    sy = qy < 0 ? bv.L[1] : (qy > bv.L[1] ? -bv.L[1] : 0.0);
+    sz = qz < 0 ? bv.L[2] : (qz > bv.L[2] ? -bv.L[2] : 0.0);
+  }
+  __syncthreads();
+  for (int k = t; k < len; k += NT) {
+    if (sx != 0.0) sh.nx[k] += sx;
+    if (sy != 0.0) sh.ny[k] += sy;
+    if (sz != 0.0) sh.nz[k] += sz;
+  }
+  __syncthreads();
+}
+
+struct MoveDraw {
+  int32_t mol;
+  int32_t vertex;
+  double d[3];
+  double cs[3];
+  double sn[3];
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
+                                              BoxCtl* ctl,
+                                              const int32_t* active,
+                                              int sweeps) {
+  __shared__ EvalShared<NT> sh;
+  __shared__ BatchBox B;
+  __shared__ Rng31 rng;
+  __shared__ MoveDraw mv;
+  __shared__ BoxCtl c;
+  const int t = threadIdx.x;
+  const int slot = active[blockIdx.x];
+  if (t == 0) {
+    B = table[slot];
+    rng = *B.rng;
+    c = ctl[slot];
+  }
+  __syncthreads();
+  const BoxView& bv = B.bv;
+  for (int s = 0; s < sweeps; s++) {
+    if (c.steps_done >= c.next_stop) break;  // block-uniform (LDS)
+    if (t == 0) {
+#pragma clang fp contract(off)
+      const int n = bv.n_mol;
+      int mol = (int)random_real(rng31_next(rng), 0, n);
+      mol = mol >= n ? n - 1 : mol;  // rand() == RAND_MAX
+      const int len = bv.mol_len[mol];
+      int vtx = (int)random_real(rng31_next(rng), 0, len);
+      vtx = vtx >= len ? len - 1 : vtx;
+      mv.mol = mol;
+      mv.vertex = vtx;
+      for (int k = 0; k < 3; k++) mv.d[k] = random_real(rng31_next(rng), -B.max_t, B.max_t);
+      for (int k = 0; k < 3; k++) {
+        const double deg = random_real(rng31_next(rng), -B.max_r, B.max_r);
+        const double rad = deg * 3.14159265358979 / 180.0;
+        mv.cs[k] = cos(rad);
+        mv.sn[k] = sin(rad);
+      }
+    }
+    __syncthreads();
+    const int mol = mv.mol;
+    load_molecule<NT>(bv, mol, sh);
+    __syncthreads();
+    propose_device<NT>(sh, bv, mv.vertex, mv.d, mv.cs, mv.sn);
+    double eo, en;
+    if (B.precision == MCGPU_MIXED)
+      eval_molecule<NT, true>(bv, mol, 0, true, sh, eo, en, nullptr);
+    else
+      eval_molecule<NT, false>(bv, mol, 0, true, sh, eo, en, nullptr);
+    if (t == 0) {
+      const double de = en - eo;
+      int acc;
+      if (de < 0) {
+        acc = 1;
+      } else {
+        acc = exp(-(de) / bv.kT) >= random_real(rng31_next(rng), 0.0, 1.0);
+      }
+      if (B.record) {
+        B.tr_mol[c.steps_done] = mol;
+        B.tr_de[c.steps_done] = de;
+        B.tr_acc[c.steps_done] = (int8_t)acc;
+      }
+      if (acc) {
+        c.energy += en - eo;
+        c.accepted++;
+      }
+      c.steps_done++;
+      sh.flag = acc;
+    }
+    __syncthreads();
+    if (sh.flag) commit_molecule<NT>(bv, mol, sh);
+    __syncthreads();
+  }
+  if (t == 0) {
+    *B.rng = rng;
+    if (c.steps_done >= c.steps_target)
+      c.status = kDone;
+    else if (c.steps_done >= c.next_stop)
+      c.status = kVolumePending;
+    ctl[slot] = c;
+  }
+}
+
+__global__ void k_rng_draw(Rng31* rng, int n, int32_t* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    Rng31 g = *rng;
+    for (int i = 0; i < n; i++) out[i] = rng31_next(g);
+    *rng = g;
+  }
+}
+
+// ------------------------------------------------------------ host side
+
+// Best-fit allocator over one device chunk (offsets, 256-byte aligned).
+class Arena {
+ public:
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  int init(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    MCGPU_HIP_TRY(hipMalloc(&base_, bytes));
+    cap_ = bytes;
+    free_[0] = bytes;
+    return MCGPU_OK;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = (bytes + 255) & ~size_t(255);
+    auto best = free_.end();
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes && (best == free_.end() || it->second < best->second))
+        best = it;
+    if (best == free_.end()) return false;
+    *off = best->first;
+    const size_t rest = best->second - bytes;
+    const size_t roff = best->first + bytes;
+    free_.erase(best);
+    if (rest) free_[roff] = rest;
+    live_[*off] = bytes;
+    used_ += bytes;
+    peak_ = std::max(peak_, used_);
+    return true;
+  }
+  void release(size_t off) {
+    auto it = live_.find(off);
+    if (it == live_.end()) return;
+    size_t size = it->second;
+    live_.erase(it);
+    used_ -= size;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + size == nx->first) {
+      size += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += size;
+        return;
+      }
+    }
+    free_[off] = size;
+  }
+  char* base() const { return (char*)base_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  void* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_, live_;
+};
+
+// Offsets of one box's arrays inside its arena block.
+struct BoxLayout {
+  size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, pidx, p64, p32;
+  size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, rng, partial,
+      pairs, total, bx, by, bz, trm, trd, tra;
+  size_t bytes;
+  int ncell, cap;
+  int cn[3];
+  double inv_w[3];
+};
+
+BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
+                      const ParamTable& pt) {
+  BoxLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (std::max<size_t>(bytes, 8) + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t ns = d.n_sites, nm = d.n_molecules, np = d.n_primary;
+  L.x = take(8 * ns);
+  L.y = take(8 * ns);
+  L.z = take(8 * ns);
+  L.px = take(8 * np);
+  L.py = take(8 * np);
+  L.pz = take(8 * np);
+  L.ms = take(4 * nm);
+  L.ml = take(4 * nm);
+  L.mps = take(4 * nm);
+  L.mpc = take(4 * nm);
+  L.mp = take(4 * nm);
+  L.pidx = take(4 * np);
+  L.p64 = take(sizeof(SiteParam64) * pt.p64.size());
+  L.p32 = take(sizeof(SiteParam32) * pt.p32.size());
+  L.ncell = 0;
+  if (p.neighbor == MCGPU_NEIGHBOR_CELL) {
+    // room for the grid of a box up to 25% larger in volume (NPT)
+    double Lg[3];
+    for (int k = 0; k < 3; k++) Lg[k] = d.box[k] * 1.08;
+    int n[3];
+    double iw[3];
+    cell_geometry(Lg, d.cutoff, pt.spread, n, iw);
+    L.ncell = n[0] * n[1] * n[2];
+    cell_geometry(d.box, d.cutoff, pt.spread, L.cn, L.inv_w);
+    const int nc_now = L.cn[0] * L.cn[1] * L.cn[2];
+    L.cap = ((int)std::ceil(3.0 * nm / nc_now) + 16 + 3) & ~3;
+    L.ccount = take(4 * (size_t)L.ncell);
+    L.cslots = take(4 * (size_t)L.ncell * L.cap);
+  }
+  L.mcell = take(4 * nm);
+  L.mslot = take(4 * nm);
+  L.covf = take(4);
+  L.vstale = take(4);
+  L.vcount = take(4);
+  L.rng = take(sizeof(Rng31));
+  L.partial = take(8 * nm);
+  L.pairs = take(8 * nm);
+  L.total = take(16);
+  if (p.ensemble == MCGPU_NPT) {
+    L.bx = take(8 * ns);
+    L.by = take(8 * ns);
+    L.bz = take(8 * ns);
+  }
+  if (p.record_trace) {
+    L.trm = take(4 * (size_t)p.steps);
+    L.trd = take(8 * (size_t)p.steps);
+    L.tra = take((size_t)p.steps);
+  }
+  L.bytes = off;
+  return L;
+}
+
+}  // namespace mcgpu
+
+using namespace mcgpu;
+
+struct BatchSlot {
+  bool used = false;
+  size_t off = 0;
+  BoxLayout lay{};
+  BatchBox box{};      // host copy of the device table entry
+  mcgpu_run_params params{};
+  int32_t n_pidx = 0;
+  double lrc = 0.0;    // current tail correction (scaled with 1/V under NPT)
+  double spread = 0.0; // largest primary-to-first-primary distance
+  double start_energy = 0.0;
+  int64_t vol_moves = 0, vol_acc = 0;
+  int fault = 0;
+};
+
+struct mcgpu_batch {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Arena arena;
+  int32_t max_boxes = 0;
+  std::vector<BatchSlot> slots;
+  BatchBox* d_table = nullptr;
+  BoxCtl* d_ctl = nullptr;
+  BoxCtl* h_ctl = nullptr;  // pinned mirror
+  int32_t* d_active = nullptr;
+  int32_t* h_active = nullptr;  // pinned
+  int32_t* d_draw = nullptr;
+  int32_t* h_draw = nullptr;  // pinned
+  double* h_total = nullptr;  // pinned [2]
+  int n_active = 0;
+  bool pending = false;  // a launch whose results were not yet consumed
+  bool ctl_dirty = false;
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_sweeps = -1;
+  int graph_active = -1;
+  uint64_t graph_version = 0, version = 1;
+  // timing
+  double kernel_ms_sum = 0.0;
+  int64_t kernel_launches = 0;
+  bool timing_pending = false;
+};
+
+namespace {
+
+int batch_sync(mcgpu_batch* b);
+
+int upload_slot(mcgpu_batch* b, int s) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table + s, &b->slots[s].box, sizeof(BatchBox),
+                               hipMemcpyHostToDevice, b->stream));
+  return MCGPU_OK;
+}
+
+int upload_ctl(mcgpu_batch* b, int s) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_ctl + s, b->h_ctl + s, sizeof(BoxCtl),
+                               hipMemcpyHostToDevice, b->stream));
+  return MCGPU_OK;
+}
+
+// Full-system energy of slot s (pairs only), synchronous.
+int slot_energy(mcgpu_batch* b, int s, double* e) {
+  BatchSlot& S = b->slots[s];
+  char* base = b->arena.base() + S.off;
+  int rc = launch_system_energy(S.box.bv, S.box.precision, (double*)(base + S.lay.partial),
+                                (double*)(base + S.lay.pairs), (double*)(base + S.lay.total),
+                                b->stream);
+  if (rc) return rc;
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_total, base + S.lay.total, 16,
+                               hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  *e = b->h_total[0];
+  return MCGPU_OK;
+}
+
+// Rebuilds slot s's cell list for its current box size; falls back to the
+// all-molecule scan when the grid no longer fits the allocation.
+int slot_rebuild_cells(mcgpu_batch* b, int s, double spread) {
+  BatchSlot& S = b->slots[s];
+  BoxView& v = S.box.bv;
+  if (S.lay.ncell == 0) {
+    v.mode = MCGPU_NB_BRUTE;
+    return MCGPU_OK;
+  }
+  int n[3];
+  double iw[3];
+  cell_geometry(v.L, std::sqrt(v.cutoff2), spread, n, iw);
+  if (n[0] * n[1] * n[2] > S.lay.ncell) {
+    v.mode = MCGPU_NB_BRUTE;
+    return MCGPU_OK;
+  }
+  for (int k = 0; k < 3; k++) {
+    v.cells.n[k] = n[k];
+    v.cells.inv_w[k] = iw[k];
+  }
+  v.mode = MCGPU_NB_CELL;
+  int rc = launch_cells_build(v, b->stream);
+  if (rc) return rc;
+  int32_t ovf = 0;
+  MCGPU_HIP_TRY(hipMemcpyAsync(&ovf, v.cells.overflow, 4, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (ovf) v.mode = MCGPU_NB_BRUTE;  // capacity exceeded: exact, slower scan
+  return MCGPU_OK;
+}
+
+// NPT volume move for slot s (build semantics; the reference's NPT path,
+// Ensemble.cpp:9-28 / SimulationStep.cpp:476-518, always grows the box by
+// 1.1 and never recomputes the energy):
+//   V' = V (1 + delta (2 u1 - 1)), molecules translated so first primaries
+//   scale by (V'/V)^(1/3); dE = dE_pairs + LRC(V') - LRC(V);
+//   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2.
+int batch_volume_move(mcgpu_batch* b, int s) {
+  BatchSlot& S = b->slots[s];
+  BoxView& v = S.box.bv;
+  char* base = b->arena.base() + S.off;
+  BoxCtl& c = b->h_ctl[s];
+  int rc;
+  double e_old;
+  if ((rc = slot_energy(b, s, &e_old))) return rc;
+  hipLaunchKernelGGL(k_rng_draw, dim3(1), dim3(64), 0, b->stream, S.box.rng, 2,
+                     b->d_draw);
+  MCGPU_HIP_TRY(hipGetLastError());
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_draw, b->d_draw, 8, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  const double u1 = random_real(b->h_draw[0], 0.0, 1.0);
+  const double u2 = random_real(b->h_draw[1], 0.0, 1.0);
+  const double v_old = v.L[0] * v.L[1] * v.L[2];
+  const double v_new = v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
+  const double f = std::cbrt(v_new / v_old);
+  const size_t nb = 8 * (size_t)v.n_site;
+  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bx, v.x, nb, hipMemcpyDeviceToDevice, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.by, v.y, nb, hipMemcpyDeviceToDevice, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bz, v.z, nb, hipMemcpyDeviceToDevice, b->stream));
+  double L_old[3];
+  for (int k = 0; k < 3; k++) L_old[k] = v.L[k];
+  if ((rc = launch_volume_scale(v, f, b->stream))) return rc;
+  for (int k = 0; k < 3; k++) v.L[k] = L_old[k] * f;
+  if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
+  if ((rc = slot_rebuild_cells(b, s, S.spread))) return rc;
+  double e_new;
+  if ((rc = slot_energy(b, s, &e_new))) return rc;
+  const double lrc_new = S.lrc * v_old / v_new;
+  const double de = (e_new - e_old) + (lrc_new - S.lrc);
+  double prob = v.n_mol * (std::log(v_new) - std::log(v_old));
+  prob -= (de + S.params.pressure * (v_new - v_old)) / v.kT;
+  prob = std::exp(prob);
+  S.vol_moves++;
+  if (prob >= u2) {
+    S.vol_acc++;
+    c.energy += de;
+    S.lrc = lrc_new;
+  } else {
+    MCGPU_HIP_TRY(hipMemcpyAsync(v.x, base + S.lay.bx, nb, hipMemcpyDeviceToDevice, b->stream));
+    MCGPU_HIP_TRY(hipMemcpyAsync(v.y, base + S.lay.by, nb, hipMemcpyDeviceToDevice, b->stream));
+    MCGPU_HIP_TRY(hipMemcpyAsync(v.z, base + S.lay.bz, nb, hipMemcpyDeviceToDevice, b->stream));
+    for (int k = 0; k < 3; k++) v.L[k] = L_old[k];
+    if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
+    if ((rc = slot_rebuild_cells(b, s, S.spread))) return rc;
+  }
+  c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
+  c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
+  if ((rc = upload_slot(b, s))) return rc;
+  if ((rc = upload_ctl(b, s))) return rc;
+  b->version++;
+  return MCGPU_OK;
+}
+
+// Consumes a finished launch: reads timing, runs pending volume moves.
+int batch_sync(mcgpu_batch* b) {
+  if (!b->pending) return MCGPU_OK;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->pending = false;
+  if (b->timing_pending) {
+    float ms = 0.f;
+    MCGPU_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->kernel_ms_sum += ms;
+    b->kernel_launches++;
+    b->timing_pending = false;
+  }
+  for (int s = 0; s < b->max_boxes; s++) {
+    if (!b->slots[s].used) continue;
+    if (b->h_ctl[s].status == kVolumePending) {
+      int rc = batch_volume_move(b, s);
+      if (rc) return rc;
+    }
+  }
+  return MCGPU_OK;
+}
+
+void free_slot(mcgpu_batch* b, int s) {
+  if (!b->slots[s].used) return;
+  b->arena.release(b->slots[s].off);
+  b->slots[s] = BatchSlot();
+  b->version++;
+}
+
+}  // namespace
+
+extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
+                                  mcgpu_batch** out) {
+  if (!out || max_boxes < 1) {
+    set_error("mcgpu_batch_create: bad argument");
+    return MCGPU_ERR_INVALID;
+  }
+  *out = nullptr;
+  int rc = ensure_device();
+  if (rc) return rc;
+  auto* b = new (std::nothrow) mcgpu_batch();
+  if (!b) return MCGPU_ERR_NOMEM;
+  b->max_boxes = max_boxes;
+  b->slots.resize(max_boxes);
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = size_t(8) << 30;
+    budget = std::min(fr / 4, size_t(64) << 30);
+  }
+  auto fail = [&](int code) {
+    mcgpu_batch_destroy(b);
+    return code;
+  };
+  if ((rc = b->arena.init(budget))) return fail(rc);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+      hipMalloc(&b->d_table, sizeof(BatchBox) * max_boxes) != hipSuccess ||
+      hipMalloc(&b->d_ctl, sizeof(BoxCtl) * max_boxes) != hipSuccess ||
+      hipMalloc(&b->d_active, sizeof(int32_t) * max_boxes) != hipSuccess ||
+      hipMalloc(&b->d_draw, 64) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * max_boxes, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * max_boxes, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_draw, 64, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_total, 16, 0) != hipSuccess) {
+    set_error("mcgpu_batch_create: allocation failed");
+    return fail(MCGPU_ERR_NOMEM);
+  }
+  std::memset(b->h_ctl, 0, sizeof(BoxCtl) * max_boxes);
+  for (int s = 0; s < max_boxes; s++) b->h_ctl[s].status = kDone;
+  if (hipMemcpy(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * max_boxes, hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    set_error("mcgpu_batch_create: upload failed");
+    return fail(MCGPU_ERR_HIP);
+  }
+  *out = b;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
+  if (!b) return MCGPU_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_table) (void)hipFree(b->d_table);
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
+  if (b->d_active) (void)hipFree(b->d_active);
+  if (b->d_draw) (void)hipFree(b->d_draw);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  if (b->h_active) (void)hipHostFree(b->h_active);
+  if (b->h_draw) (void)hipHostFree(b->h_draw);
+  if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_footprint(const mcgpu_box_desc* d,
+                                   const mcgpu_run_params* p, size_t* bytes) {
+  if (!d || !p || !bytes) return MCGPU_ERR_INVALID;
+  const std::string why = validate_desc(*d);
+  if (!why.empty()) {
+    set_error("mcgpu_box_footprint: " + why);
+    return MCGPU_ERR_INVALID;
+  }
+  const ParamTable pt = build_param_table(*d);
+  *bytes = plan_layout(*d, *p, pt).bytes;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
+                               const mcgpu_run_params* p, int32_t* slot_out) {
+  if (!b || !d || !p || !slot_out) return MCGPU_ERR_INVALID;
+  const std::string why = validate_desc(*d);
+  if (!why.empty()) {
+    set_error("mcgpu_batch_add: " + why);
+    return MCGPU_ERR_INVALID;
+  }
+  if ((p->precision != MCGPU_FP64 && p->precision != MCGPU_MIXED) ||
+      (p->ensemble != MCGPU_NVT && p->ensemble != MCGPU_NPT) || p->steps < 0 ||
+      (p->neighbor != MCGPU_NEIGHBOR_BRUTE && p->neighbor != MCGPU_NEIGHBOR_CELL) ||
+      (p->ensemble == MCGPU_NPT && (p->volume_interval < 1 || !(p->max_volume_delta > 0) ||
+                                    !(p->max_volume_delta < 0.5)))) {
+    set_error("mcgpu_batch_add: bad run parameters");
+    return MCGPU_ERR_INVALID;
+  }
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  int s = -1;
+  for (int k = 0; k < b->max_boxes; k++)
+    if (!b->slots[k].used) {
+      s = k;
+      break;
+    }
+  if (s < 0) {
+    set_error("mcgpu_batch_add: no free slot");
+    return MCGPU_ERR_NOMEM;
+  }
+  const ParamTable pt = build_param_table(*d);
+  const BoxLayout L = plan_layout(*d, *p, pt);
+  size_t off = 0;
+  if (!b->arena.alloc(L.bytes, &off)) {
+    set_error("mcgpu_batch_add: HBM budget exhausted");
+    return MCGPU_ERR_NOMEM;
+  }
+  BatchSlot& S = b->slots[s];
+  S.used = true;
+  S.off = off;
+  S.lay = L;
+  S.params = *p;
+  S.n_pidx = d->n_primary;
+  S.lrc = p->energy_offset;
+  S.spread = pt.spread;
+  char* base = b->arena.base() + off;
+  BoxView& v = S.box.bv;
+  v = BoxView{};
+  v.x = (double*)(base + L.x);
+  v.y = (double*)(base + L.y);
+  v.z = (double*)(base + L.z);
+  v.px = (double*)(base + L.px);
+  v.py = (double*)(base + L.py);
+  v.pz = (double*)(base + L.pz);
+  v.mol_start = (int32_t*)(base + L.ms);
+  v.mol_len = (int32_t*)(base + L.ml);
+  v.mol_pidx_start = (int32_t*)(base + L.mps);
+  v.mol_pidx_count = (int32_t*)(base + L.mpc);
+  v.mol_param = (int32_t*)(base + L.mp);
+  v.pidx = (int32_t*)(base + L.pidx);
+  v.p64 = (SiteParam64*)(base + L.p64);
+  v.p32 = (SiteParam32*)(base + L.p32);
+  v.n_mol = d->n_molecules;
+  v.n_site = d->n_sites;
+  v.max_len = pt.max_len;
+  v.mode = MCGPU_NB_BRUTE;
+  for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
+  v.cutoff2 = d->cutoff * d->cutoff;
+  v.kT = d->kT;
+  v.cells.count = (int32_t*)(base + L.ccount);
+  v.cells.slots = (int32_t*)(base + L.cslots);
+  v.cells.mol_cell = (int32_t*)(base + L.mcell);
+  v.cells.mol_slot = (int32_t*)(base + L.mslot);
+  v.cells.overflow = (int32_t*)(base + L.covf);
+  v.cells.cap = L.cap;
+  v.verlet.stale = (int32_t*)(base + L.vstale);
+  v.verlet.count = (int32_t*)(base + L.vcount);
+  S.box.rng = (Rng31*)(base + L.rng);
+  S.box.max_t = d->max_translate;
+  S.box.max_r = d->max_rotate;
+  S.box.precision = p->precision;
+  S.box.record = p->record_trace ? 1 : 0;
+  S.box.tr_mol = (int32_t*)(base + L.trm);
+  S.box.tr_de = (double*)(base + L.trd);
+  S.box.tr_acc = (int8_t*)(base + L.tra);
+  Rng31 g;
+  rng31_seed(g, p->seed);
+  const int32_t zero[3] = {0, 0, 0};
+  struct Copy {
+    size_t off;
+    const void* src;
+    size_t bytes;
+  } copies[] = {{L.x, d->x, 8 * (size_t)d->n_sites},
+                {L.y, d->y, 8 * (size_t)d->n_sites},
+                {L.z, d->z, 8 * (size_t)d->n_sites},
+                {L.ms, d->mol_start, 4 * (size_t)d->n_molecules},
+                {L.ml, d->mol_len, 4 * (size_t)d->n_molecules},
+                {L.mps, d->mol_pidx_start, 4 * (size_t)d->n_molecules},
+                {L.mpc, d->mol_pidx_count, 4 * (size_t)d->n_molecules},
+                {L.mp, pt.mol_param.data(), 4 * (size_t)d->n_molecules},
+                {L.pidx, d->primary_index, 4 * (size_t)d->n_primary},
+                {L.p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
+                {L.p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()},
+                {L.covf, zero, 4},
+                {L.vstale, zero, 4},
+                {L.vcount, zero, 4},
+                {L.rng, &g, sizeof(Rng31)}};
+  for (const Copy& c : copies) {
+    if (hipMemcpyAsync(base + c.off, c.src, c.bytes, hipMemcpyHostToDevice, b->stream) !=
+        hipSuccess) {
+      free_slot(b, s);
+      set_error("mcgpu_batch_add: upload failed");
+      return MCGPU_ERR_HIP;
+    }
+  }
+  // The host arrays are read by the copies above: finish before returning.
+  if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream)) ||
+      (p->neighbor == MCGPU_NEIGHBOR_CELL && (rc = slot_rebuild_cells(b, s, pt.spread)))) {
+    free_slot(b, s);
+    return rc;
+  }
+  double e0 = 0.0;
+  if ((rc = slot_energy(b, s, &e0))) {
+    free_slot(b, s);
+    return rc;
+  }
+  S.start_energy = e0 + p->energy_offset;
+  BoxCtl& c = b->h_ctl[s];
+  c = BoxCtl{};
+  c.steps_target = p->steps;
+  c.energy = S.start_energy;
+  c.next_stop = (p->ensemble == MCGPU_NPT) ? std::min<int64_t>(p->steps, p->volume_interval)
+                                           : p->steps;
+  c.status = p->steps > 0 ? kRunning : kDone;
+  if ((rc = upload_slot(b, s)) || (rc = upload_ctl(b, s))) {
+    free_slot(b, s);
+    return rc;
+  }
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->version++;
+  *slot_out = s;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
+  if (!b || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
+    set_error("mcgpu_batch_remove: bad slot");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  b->h_ctl[s].status = kDone;
+  if ((rc = upload_ctl(b, s))) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  free_slot(b, s);
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
+  if (!b || sweeps < 1) return MCGPU_ERR_INVALID;
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  int n = 0;
+  for (int s = 0; s < b->max_boxes; s++)
+    if (b->slots[s].used && b->h_ctl[s].status == kRunning) b->h_active[n++] = s;
+  if (n == 0) return MCGPU_OK;
+  const bool same = b->gexec && b->graph_version == b->version &&
+                    b->graph_sweeps == sweeps && b->graph_active == n;
+  if (!same) {
+    if (b->gexec) MCGPU_HIP_TRY(hipGraphExecDestroy(b->gexec));
+    if (b->graph) MCGPU_HIP_TRY(hipGraphDestroy(b->graph));
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active, b->h_active, sizeof(int32_t) * n,
+                                 hipMemcpyHostToDevice, b->stream));
+    // The persistent launch of one tick is captured once and replayed
+    // while the box set and sweep count are unchanged.
+    MCGPU_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_batch<256>, dim3(n), dim3(256), 0, b->stream, b->d_table,
+                       b->d_ctl, b->d_active, sweeps);
+    MCGPU_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+    MCGPU_HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->graph_version = b->version;
+    b->graph_sweeps = sweeps;
+    b->graph_active = n;
+  }
+  MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+  MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
+                               hipMemcpyDeviceToHost, b->stream));
+  b->n_active = n;
+  b->pending = true;
+  b->timing_pending = true;
+  if (sync) return batch_sync(b);
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out) {
+  if (!b || !out || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
+    set_error("mcgpu_batch_stats: bad slot");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  const BoxCtl& c = b->h_ctl[s];
+  out->steps_done = c.steps_done;
+  out->accepted = c.accepted;
+  out->volume_moves = b->slots[s].vol_moves;
+  out->volume_accepted = b->slots[s].vol_acc;
+  out->energy = c.energy;
+  out->start_energy = b->slots[s].start_energy;
+  for (int k = 0; k < 3; k++) out->box[k] = b->slots[s].box.bv.L[k];
+  out->done = c.status == kDone;
+  out->fault = b->slots[s].fault;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_download(mcgpu_batch* b, int32_t s, double* x,
+                                    double* y, double* z) {
+  if (!b || s < 0 || s >= b->max_boxes || !b->slots[s].used || !x || !y || !z) {
+    set_error("mcgpu_batch_download: bad argument");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  const BoxView& v = b->slots[s].box.bv;
+  const size_t nb = 8 * (size_t)v.n_site;
+  MCGPU_HIP_TRY(hipMemcpyAsync(x, v.x, nb, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(y, v.y, nb, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(z, v.z, nb, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_trace(mcgpu_batch* b, int32_t s, int32_t* mol,
+                                 double* delta, int8_t* acc, int64_t cap,
+                                 int64_t* n) {
+  if (!b || s < 0 || s >= b->max_boxes || !b->slots[s].used || !n) {
+    set_error("mcgpu_batch_trace: bad argument");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  const BatchSlot& S = b->slots[s];
+  if (!S.box.record) {
+    set_error("mcgpu_batch_trace: box was added without record_trace");
+    return MCGPU_ERR_STATE;
+  }
+  const int64_t k = std::min<int64_t>(cap, b->h_ctl[s].steps_done);
+  if (k > 0) {
+    if (mol) MCGPU_HIP_TRY(hipMemcpyAsync(mol, S.box.tr_mol, 4 * k, hipMemcpyDeviceToHost, b->stream));
+    if (delta) MCGPU_HIP_TRY(hipMemcpyAsync(delta, S.box.tr_de, 8 * k, hipMemcpyDeviceToHost, b->stream));
+    if (acc) MCGPU_HIP_TRY(hipMemcpyAsync(acc, S.box.tr_acc, k, hipMemcpyDeviceToHost, b->stream));
+    MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  }
+  *n = k;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms,
+                                     int64_t* launches) {
+  if (!b || !avg_ms || !launches) return MCGPU_ERR_INVALID;
+  int rc = batch_sync(b);
+  if (rc) return rc;
+  *launches = b->kernel_launches;
+  *avg_ms = b->kernel_launches ? b->kernel_ms_sum / b->kernel_launches : 0.0;
+  b->kernel_ms_sum = 0.0;
+  b->kernel_launches = 0;
+  return MCGPU_OK;
+}

@@ -1,0 +1,878 @@
+// Single device-resident box: residency (replaces GPUCopy, GPUCopy.cpp:97-222),
+// the molecule-energy and fused trial-move kernels (replace the BruteForceStep
+// OpenACC region, BruteForceStep.cpp:31-76, and the move/rollback regions,
+// SimulationStep.cpp:545-565,738-744), the full-system energy
+// (SimulationStep.cpp:40-71), neighbour rebuilds (NLCStep.cpp:125-253,
+// VerletStep.cpp:119-207, done correctly) and the NPT box rescale.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "box_internal.h"
+#include "eval.h"
+
+namespace mcgpu {
+
+// ------------------------------------------------------------- kernels
+
+struct MoveArgs {
+  int32_t mol;
+  int32_t start_mol;
+  int32_t has_new;
+  int32_t metropolis;  // apply the Metropolis test with u and commit
+  double u;
+  double nxyz[3 * kMaxSites];  // x block, y block, z block
+};
+
+template <int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_move(BoxView bv, MoveArgs a,
+                                             MoveOut* out) {
+  __shared__ EvalShared<NT> sh;
+  const int t = threadIdx.x;
+  load_molecule<NT>(bv, a.mol, sh);
+  const int len = bv.mol_len[a.mol];
+  if (a.has_new) {
+    for (int k = t; k < len; k += NT) {
+      sh.nx[k] = a.nxyz[k];
+      sh.ny[k] = a.nxyz[len + k];
+      sh.nz[k] = a.nxyz[2 * len + k];
+    }
+  }
+  __syncthreads();
+  double eo, en;
+  eval_molecule<NT, MIXED>(bv, a.mol, a.start_mol, a.has_new != 0, sh, eo, en,
+                           nullptr);
+  if (t == 0) {
+    const double de = en - eo;
+    int acc = 0;
+    if (a.metropolis) acc = (de < 0) || (exp(-(de) / bv.kT) >= a.u);
+    out->e_old = eo;
+    out->e_new = en;
+    out->de = de;
+    out->accepted = acc;
+    out->source = bv.mode;
+    sh.flag = acc;
+  }
+  __syncthreads();
+  if (a.metropolis && sh.flag) commit_molecule<NT>(bv, a.mol, sh);
+}
+
+// Overwrite one molecule (changeMolecule / rollback from the host).
+__global__ __launch_bounds__(64) void k_set_mol(BoxView bv, MoveArgs a) {
+  __shared__ EvalShared<64> sh;
+  const int t = threadIdx.x;
+  load_molecule<64>(bv, a.mol, sh);
+  const int len = bv.mol_len[a.mol];
+  for (int k = t; k < len; k += 64) {
+    sh.nx[k] = a.nxyz[k];
+    sh.ny[k] = a.nxyz[len + k];
+    sh.nz[k] = a.nxyz[2 * len + k];
+  }
+  __syncthreads();
+  commit_molecule<64>(bv, a.mol, sh);
+}
+
+__global__ void k_refresh_primary(BoxView bv, int n_pidx) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_pidx) {
+    const int s = bv.pidx[k];
+    bv.px[k] = bv.x[s];
+    bv.py[k] = bv.y[s];
+    bv.pz[k] = bv.z[s];
+  }
+}
+
+__global__ void k_cells_clear(CellGrid g, int ncell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell) g.count[c] = 0;
+  if (c == 0) g.overflow[0] = 0;
+}
+
+__global__ void k_cells_assign(BoxView bv) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= bv.n_mol) return;
+  const CellGrid& g = bv.cells;
+  const int p = bv.mol_pidx_start[m];
+  const int c = cell_index(g, cell_coord(bv.px[p], g.inv_w[0], g.n[0]),
+                           cell_coord(bv.py[p], g.inv_w[1], g.n[1]),
+                           cell_coord(bv.pz[p], g.inv_w[2], g.n[2]));
+  const int slot = atomicAdd(&g.count[c], 1);
+  g.mol_cell[m] = c;
+  if (slot < g.cap) {
+    g.slots[c * g.cap + slot] = m;
+  } else {
+    atomicOr(g.overflow, 1);
+  }
+}
+
+// Sort each cell's slots by molecule id so the candidate order (and so the
+// summation order) does not depend on atomic arrival order.
+__global__ void k_cells_sort(CellGrid g, int ncell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int n = g.count[c];
+  n = n < g.cap ? n : g.cap;
+  int32_t* s = g.slots + (size_t)c * g.cap;
+  for (int i = 1; i < n; i++) {
+    const int v = s[i];
+    int k = i - 1;
+    while (k >= 0 && s[k] > v) {
+      s[k + 1] = s[k];
+      k--;
+    }
+    s[k + 1] = v;
+  }
+  for (int i = 0; i < n; i++) g.mol_slot[s[i]] = i;
+}
+
+// Verlet list of molecule i: every j whose first primary lies within rv of
+// i's (minimum image), in increasing j.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_verlet_build(BoxView bv, double rv2) {
+  __shared__ int32_t scan[NT / 64];
+  const VerletList& v = bv.verlet;
+  for (int i = blockIdx.x; i < bv.n_mol; i += gridDim.x) {
+    const int pi = bv.mol_pidx_start[i];
+    const double xi = bv.px[pi], yi = bv.py[pi], zi = bv.pz[pi];
+    if (threadIdx.x == 0) {
+      v.ref[3 * i] = xi;
+      v.ref[3 * i + 1] = yi;
+      v.ref[3 * i + 2] = zi;
+    }
+    int n = 0;
+    for (int base = 0; base < bv.n_mol; base += NT) {
+      const int j = base + threadIdx.x;
+      bool f = false;
+      if (j < bv.n_mol && j != i) {
+        const int pj = bv.mol_pidx_start[j];
+        const double dx = wrap1(bv.px[pj] - xi, bv.L[0]);
+        const double dy = wrap1(bv.py[pj] - yi, bv.L[1]);
+        const double dz = wrap1(bv.pz[pj] - zi, bv.L[2]);
+        f = dx * dx + dy * dy + dz * dz <= rv2;
+      }
+      int total;
+      const int pos = block_scan_flag<NT>(f, scan, total);
+      if (f && n + pos < v.cap) v.list[(size_t)i * v.cap + n + pos] = j;
+      n += total;
+    }
+    if (threadIdx.x == 0) {
+      v.count[i] = n;
+      if (n > v.cap) atomicOr(v.stale, 2);  // capacity exceeded
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_system(BoxView bv, double* partial,
+                                               double* pairs) {
+  __shared__ EvalShared<NT> sh;
+  for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
+    load_molecule<NT>(bv, mol, sh);
+    __syncthreads();
+    double eo, en;
+    unsigned long long np = 0;
+    eval_molecule<NT, MIXED>(bv, mol, mol + 1, false, sh, eo, en, &np);
+    if (threadIdx.x == 0) {
+      partial[mol] = eo;
+      pairs[mol] = (double)np;
+    }
+    __syncthreads();
+  }
+}
+
+// Fixed-order sum of n partials into out[0] (and n counts into out[1]).
+__global__ __launch_bounds__(256) void k_reduce2(const double* a,
+                                                 const double* b, int n,
+                                                 double* out) {
+  __shared__ double red[4];
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    s += a[i];
+    c += b[i];
+  }
+  s = block_sum<256>(s, red);
+  c = block_sum<256>(c, red);
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = c;
+  }
+}
+
+// NPT rescale: every molecule translated so its first primary moves from p
+// to f * p (internal geometry untouched).
+__global__ void k_volume_scale(BoxView bv, double f) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= bv.n_mol) return;
+  const int p = bv.pidx[bv.mol_pidx_start[m]];
+  const double sx = bv.x[p] * f - bv.x[p];
+  const double sy = bv.y[p] * f - bv.y[p];
+  const double sz = bv.z[p] * f - bv.z[p];
+  const int s0 = bv.mol_start[m], s1 = s0 + bv.mol_len[m];
+  for (int s = s0; s < s1; s++) {
+    bv.x[s] += sx;
+    bv.y[s] += sy;
+    bv.z[s] += sz;
+  }
+}
+
+// ------------------------------------------------------ launch helpers
+
+int launch_refresh_primary(const BoxView& bv, int n_pidx, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_primary, dim3((n_pidx + 255) / 256), dim3(256), 0,
+                     s, bv, n_pidx);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int launch_cells_build(const BoxView& bv, hipStream_t s) {
+  const CellGrid& g = bv.cells;
+  const int ncell = g.n[0] * g.n[1] * g.n[2];
+  hipLaunchKernelGGL(k_cells_clear, dim3((ncell + 255) / 256), dim3(256), 0, s,
+                     g, ncell);
+  hipLaunchKernelGGL(k_cells_assign, dim3((bv.n_mol + 255) / 256), dim3(256), 0,
+                     s, bv);
+  hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 255) / 256), dim3(256), 0, s,
+                     g, ncell);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int launch_system_energy(const BoxView& bv, int precision, double* d_partial,
+                         double* d_pairs, double* d_out2, hipStream_t s) {
+  const int n = bv.n_mol;
+  const int grid = std::min(n, 8192);
+  if (precision == MCGPU_MIXED)
+    hipLaunchKernelGGL((k_system<64, true>), dim3(grid), dim3(64), 0, s, bv,
+                       d_partial, d_pairs);
+  else
+    hipLaunchKernelGGL((k_system<64, false>), dim3(grid), dim3(64), 0, s, bv,
+                       d_partial, d_pairs);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, s, d_partial, d_pairs, n,
+                     d_out2);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int launch_volume_scale(const BoxView& bv, double f, hipStream_t s) {
+  hipLaunchKernelGGL(k_volume_scale, dim3((bv.n_mol + 255) / 256), dim3(256), 0,
+                     s, bv, f);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+// ------------------------------------------------------------ host side
+
+int ensure_device() {
+  static int state = -1;  // -1 unknown, 0 ok, >0 error
+  if (state == 0) return MCGPU_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    set_error("no HIP device visible (the MI355X energy path needs a GPU)");
+    return MCGPU_ERR_NODEVICE;
+  }
+  state = 0;
+  return MCGPU_OK;
+}
+
+ParamTable build_param_table(const mcgpu_box_desc& d) {
+  ParamTable t;
+  t.mol_param.resize(d.n_molecules);
+  std::map<std::vector<uint64_t>, int32_t> sig;
+  for (int32_t m = 0; m < d.n_molecules; m++) {
+    const int32_t s0 = d.mol_start[m], len = d.mol_len[m];
+    t.max_len = std::max(t.max_len, len);
+    std::vector<uint64_t> key;
+    key.reserve(1 + 3 * len);
+    key.push_back((uint64_t)len);
+    for (int32_t k = 0; k < len; k++) {
+      uint64_t a, b, c;
+      std::memcpy(&a, &d.sigma[s0 + k], 8);
+      std::memcpy(&b, &d.epsilon[s0 + k], 8);
+      std::memcpy(&c, &d.charge[s0 + k], 8);
+      key.push_back(a);
+      key.push_back(b);
+      key.push_back(c);
+    }
+    auto it = sig.find(key);
+    if (it == sig.end()) {
+      const int32_t base = (int32_t)t.p64.size();
+      for (int32_t k = 0; k < len; k++) {
+        const double sg = d.sigma[s0 + k], ep = d.epsilon[s0 + k];
+        const bool real = sg >= 0 && ep >= 0;  // BruteForceStep.cpp:64-65
+        SiteParam64 p;
+        p.ss = real ? std::sqrt(sg) : 0.0;
+        p.se = real ? std::sqrt(ep) : 0.0;
+        p.q = real ? d.charge[s0 + k] : 0.0;
+        p.real = real ? 1.0 : 0.0;
+        t.p64.push_back(p);
+        t.p32.push_back(SiteParam32{(float)p.ss, (float)p.se, (float)p.q, (float)p.real});
+      }
+      it = sig.emplace(key, base).first;
+    }
+    t.mol_param[m] = it->second;
+    // primary spread (rigid molecules: fixed)
+    const int32_t p0 = d.primary_index[d.mol_pidx_start[m]];
+    for (int32_t k = 1; k < d.mol_pidx_count[m]; k++) {
+      const int32_t p = d.primary_index[d.mol_pidx_start[m] + k];
+      const double dx = d.x[p] - d.x[p0], dy = d.y[p] - d.y[p0], dz = d.z[p] - d.z[p0];
+      t.spread = std::max(t.spread, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  return t;
+}
+
+void cell_geometry(const double L[3], double cutoff, double spread, int n[3],
+                   double inv_w[3]) {
+  const double w = cutoff + 2.0 * spread + 1e-9;
+  for (int k = 0; k < 3; k++) {
+    n[k] = std::max(1, (int)std::floor(L[k] / w));
+    inv_w[k] = n[k] / L[k];
+  }
+}
+
+}  // namespace mcgpu
+
+using namespace mcgpu;
+
+struct mcgpu_box {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int precision = MCGPU_FP64;
+  BoxView bv{};
+  int32_t n_pidx = 0;
+  double spread = 0.0;
+  double skin = 0.0;
+  std::vector<void*> allocs;  // device allocations owned by the box
+  int32_t* cell_count = nullptr;
+  int32_t* cell_slots = nullptr;
+  int32_t ncell_alloc = 0;
+  int32_t slots_alloc = 0;
+  MoveOut* h_out = nullptr;  // pinned, mapped
+  MoveOut* d_out = nullptr;
+  double* d_partial = nullptr;
+  double* d_pairs = nullptr;
+  double* h_total = nullptr;  // pinned, mapped [2]
+  double* d_total = nullptr;
+  int64_t last_pairs = 0;
+  float last_ms = 0.0f;
+  std::vector<int32_t> h_mol_start, h_mol_len;
+};
+
+namespace {
+
+template <class T>
+int dalloc(mcgpu_box* b, T** p, size_t count) {
+  void* q = nullptr;
+  MCGPU_HIP_TRY(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+  b->allocs.push_back(q);
+  *p = (T*)q;
+  return MCGPU_OK;
+}
+
+void free_box(mcgpu_box* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->cell_count) (void)hipFree(b->cell_count);
+  if (b->cell_slots) (void)hipFree(b->cell_slots);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int check_mol(mcgpu_box* b, int32_t mol) {
+  if (!b) {
+    set_error("null box");
+    return MCGPU_ERR_INVALID;
+  }
+  if (mol < 0 || mol >= b->bv.n_mol) {
+    set_error("molecule index out of range");
+    return MCGPU_ERR_RANGE;
+  }
+  return MCGPU_OK;
+}
+
+int refresh_primary(mcgpu_box* b) {
+  const int n = b->n_pidx;
+  hipLaunchKernelGGL(k_refresh_primary, dim3((n + 255) / 256), dim3(256), 0,
+                     b->stream, b->bv, n);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int build_cells(mcgpu_box* b) {
+  CellGrid& g = b->bv.cells;
+  cell_geometry(b->bv.L, std::sqrt(b->bv.cutoff2), b->spread, g.n, g.inv_w);
+  const int ncell = g.n[0] * g.n[1] * g.n[2];
+  if (ncell > b->ncell_alloc) {
+    if (b->cell_count) MCGPU_HIP_TRY(hipFree(b->cell_count));
+    b->cell_count = nullptr;
+    MCGPU_HIP_TRY(hipMalloc(&b->cell_count, sizeof(int32_t) * ncell));
+    b->ncell_alloc = ncell;
+  }
+  g.count = b->cell_count;
+  int cap = std::max(8, (int)std::ceil(2.0 * b->bv.n_mol / ncell) + 8);
+  cap = (cap + 3) & ~3;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if ((size_t)ncell * cap > (size_t)b->slots_alloc) {
+      if (b->cell_slots) MCGPU_HIP_TRY(hipFree(b->cell_slots));
+      b->cell_slots = nullptr;
+      MCGPU_HIP_TRY(hipMalloc(&b->cell_slots, sizeof(int32_t) * (size_t)ncell * cap));
+      b->slots_alloc = ncell * cap;
+    }
+    g.cap = cap;
+    g.slots = b->cell_slots;
+    hipLaunchKernelGGL(k_cells_clear, dim3((ncell + 255) / 256), dim3(256), 0,
+                       b->stream, g, ncell);
+    hipLaunchKernelGGL(k_cells_assign, dim3((b->bv.n_mol + 255) / 256), dim3(256),
+                       0, b->stream, b->bv);
+    MCGPU_HIP_TRY(hipGetLastError());
+    int32_t ovf = 0;
+    MCGPU_HIP_TRY(hipMemcpyAsync(&ovf, g.overflow, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, b->stream));
+    MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!ovf) {
+      hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 255) / 256), dim3(256), 0,
+                         b->stream, g, ncell);
+      MCGPU_HIP_TRY(hipGetLastError());
+      return MCGPU_OK;
+    }
+    std::vector<int32_t> counts(ncell);
+    MCGPU_HIP_TRY(hipMemcpy(counts.data(), g.count, sizeof(int32_t) * ncell,
+                            hipMemcpyDeviceToHost));
+    const int mx = *std::max_element(counts.begin(), counts.end());
+    cap = ((int)(mx * 1.5) + 4 + 3) & ~3;
+  }
+  set_error("cell list build did not converge");
+  return MCGPU_ERR_STATE;
+}
+
+int build_verlet(mcgpu_box* b) {
+  VerletList& v = b->bv.verlet;
+  const double rv = std::sqrt(b->bv.cutoff2) + 2.0 * b->spread + b->skin;
+  v.skin_half2 = 0.25 * b->skin * b->skin;
+  for (int attempt = 0; attempt < 6; attempt++) {
+    MCGPU_HIP_TRY(hipMemsetAsync(v.stale, 0, sizeof(int32_t), b->stream));
+    const int grid = std::min(b->bv.n_mol, 4096);
+    hipLaunchKernelGGL(k_verlet_build<256>, dim3(grid), dim3(256), 0, b->stream,
+                       b->bv, rv * rv);
+    MCGPU_HIP_TRY(hipGetLastError());
+    int32_t st = 0;
+    MCGPU_HIP_TRY(hipMemcpyAsync(&st, v.stale, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, b->stream));
+    MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!st) return MCGPU_OK;
+    std::vector<int32_t> cnt(b->bv.n_mol);
+    MCGPU_HIP_TRY(hipMemcpy(cnt.data(), v.count, sizeof(int32_t) * cnt.size(),
+                            hipMemcpyDeviceToHost));
+    const int mx = *std::max_element(cnt.begin(), cnt.end());
+    const int cap = ((int)(mx * 1.25) + 8 + 7) & ~7;
+    // grow the list storage
+    for (auto it = b->allocs.begin(); it != b->allocs.end(); ++it)
+      if (*it == (void*)v.list) {
+        b->allocs.erase(it);
+        break;
+      }
+    MCGPU_HIP_TRY(hipFree(v.list));
+    v.list = nullptr;
+    if (dalloc(b, &v.list, (size_t)b->bv.n_mol * cap) != MCGPU_OK) return MCGPU_ERR_HIP;
+    v.cap = cap;
+  }
+  set_error("Verlet list build did not converge");
+  return MCGPU_ERR_STATE;
+}
+
+int rebuild_neighbors(mcgpu_box* b) {
+  if (b->bv.mode == MCGPU_NB_CELL) return build_cells(b);
+  if (b->bv.mode == MCGPU_NB_VERLET) return build_verlet(b);
+  return MCGPU_OK;
+}
+
+int launch_move(mcgpu_box* b, const MoveArgs& a) {
+  const bool big = (b->bv.mode == MCGPU_NB_BRUTE) && b->bv.n_mol > 2048;
+  const bool mixed = b->precision == MCGPU_MIXED;
+  if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  if (big) {
+    if (mixed)
+      hipLaunchKernelGGL((k_move<1024, true>), dim3(1), dim3(1024), 0, b->stream, b->bv, a, b->d_out);
+    else
+      hipLaunchKernelGGL((k_move<1024, false>), dim3(1), dim3(1024), 0, b->stream, b->bv, a, b->d_out);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL((k_move<256, true>), dim3(1), dim3(256), 0, b->stream, b->bv, a, b->d_out);
+    else
+      hipLaunchKernelGGL((k_move<256, false>), dim3(1), dim3(256), 0, b->stream, b->bv, a, b->d_out);
+  }
+  MCGPU_HIP_TRY(hipGetLastError());
+  if (b->ev1) MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->ev0) MCGPU_HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  return MCGPU_OK;
+}
+
+int fill_coords(mcgpu_box* b, int32_t mol, const double* xyz, MoveArgs& a) {
+  const int len = b->h_mol_len[mol];
+  for (int k = 0; k < 3 * len; k++) {
+    if (!std::isfinite(xyz[k])) {
+      set_error("non-finite coordinate");
+      return MCGPU_ERR_INVALID;
+    }
+    a.nxyz[k] = xyz[k];
+  }
+  return MCGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int mcgpu_device_count(int* count) {
+  if (!count) return MCGPU_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_device_synchronize(void) {
+  MCGPU_HIP_TRY(hipDeviceSynchronize());
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
+                                mcgpu_box** out) {
+  if (!d || !out) {
+    set_error("mcgpu_box_create: null argument");
+    return MCGPU_ERR_INVALID;
+  }
+  *out = nullptr;
+  const std::string why = validate_desc(*d);
+  if (!why.empty()) {
+    set_error("mcgpu_box_create: " + why);
+    return MCGPU_ERR_INVALID;
+  }
+  if (precision != MCGPU_FP64 && precision != MCGPU_MIXED) {
+    set_error("mcgpu_box_create: precision must be MCGPU_FP64 or MCGPU_MIXED");
+    return MCGPU_ERR_INVALID;
+  }
+  int rc = ensure_device();
+  if (rc) return rc;
+  auto* b = new (std::nothrow) mcgpu_box();
+  if (!b) return MCGPU_ERR_NOMEM;
+  b->precision = precision;
+  const ParamTable pt = build_param_table(*d);
+  b->spread = pt.spread;
+  b->n_pidx = d->n_primary;
+  b->h_mol_start.assign(d->mol_start, d->mol_start + d->n_molecules);
+  b->h_mol_len.assign(d->mol_len, d->mol_len + d->n_molecules);
+  BoxView& v = b->bv;
+  v.n_mol = d->n_molecules;
+  v.n_site = d->n_sites;
+  v.max_len = pt.max_len;
+  v.mode = MCGPU_NB_BRUTE;
+  for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
+  v.cutoff2 = d->cutoff * d->cutoff;
+  v.kT = d->kT;
+  const size_t ns = d->n_sites, nm = d->n_molecules, np = d->n_primary;
+#define ALLOC(p, n)                                           \
+  if ((rc = dalloc(b, &p, n)) != MCGPU_OK) {                  \
+    free_box(b);                                              \
+    return rc;                                                \
+  }
+  double *x, *y, *z, *px, *py, *pz;
+  int32_t *ms, *ml, *mps, *mpc, *mp, *pi;
+  SiteParam64* p64;
+  SiteParam32* p32;
+  ALLOC(x, ns);
+  ALLOC(y, ns);
+  ALLOC(z, ns);
+  ALLOC(px, np);
+  ALLOC(py, np);
+  ALLOC(pz, np);
+  ALLOC(ms, nm);
+  ALLOC(ml, nm);
+  ALLOC(mps, nm);
+  ALLOC(mpc, nm);
+  ALLOC(mp, nm);
+  ALLOC(pi, np);
+  ALLOC(p64, pt.p64.size());
+  ALLOC(p32, pt.p32.size());
+  ALLOC(v.cells.mol_cell, nm);
+  ALLOC(v.cells.mol_slot, nm);
+  ALLOC(v.cells.overflow, 1);
+  ALLOC(v.verlet.count, nm);
+  ALLOC(v.verlet.ref, 3 * nm);
+  ALLOC(v.verlet.stale, 1);
+  v.verlet.cap = 64;
+  ALLOC(v.verlet.list, nm * 64);
+  ALLOC(b->d_partial, nm);
+  ALLOC(b->d_pairs, nm);
+#undef ALLOC
+  v.x = x;
+  v.y = y;
+  v.z = z;
+  v.px = px;
+  v.py = py;
+  v.pz = pz;
+  v.mol_start = ms;
+  v.mol_len = ml;
+  v.mol_pidx_start = mps;
+  v.mol_pidx_count = mpc;
+  v.mol_param = mp;
+  v.pidx = pi;
+  v.p64 = p64;
+  v.p32 = p32;
+  auto fail = [&](int code) {
+    free_box(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(MCGPU_ERR_HIP);
+  }
+  if (hipHostMalloc((void**)&b->h_out, sizeof(MoveOut), hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&b->d_out, b->h_out, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_total, 2 * sizeof(double), hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&b->d_total, b->h_total, 0) != hipSuccess) {
+    set_error("pinned result buffer allocation failed");
+    return fail(MCGPU_ERR_NOMEM);
+  }
+  struct Copy {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } copies[] = {{x, d->x, 8 * ns},
+                {y, d->y, 8 * ns},
+                {z, d->z, 8 * ns},
+                {ms, d->mol_start, 4 * nm},
+                {ml, d->mol_len, 4 * nm},
+                {mps, d->mol_pidx_start, 4 * nm},
+                {mpc, d->mol_pidx_count, 4 * nm},
+                {mp, pt.mol_param.data(), 4 * nm},
+                {pi, d->primary_index, 4 * np},
+                {p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
+                {p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()}};
+  for (const Copy& c : copies) {
+    if (hipMemcpy(c.dst, c.src, c.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("mcgpu_box_create: upload failed");
+      return fail(MCGPU_ERR_HIP);
+    }
+  }
+  if (hipMemset(v.cells.overflow, 0, 4) != hipSuccess ||
+      hipMemset(v.verlet.stale, 0, 4) != hipSuccess ||
+      hipMemset(v.verlet.count, 0, 4 * nm) != hipSuccess) {
+    set_error("mcgpu_box_create: memset failed");
+    return fail(MCGPU_ERR_HIP);
+  }
+  if ((rc = refresh_primary(b)) != MCGPU_OK) return fail(rc);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) {
+    set_error("mcgpu_box_create: synchronize failed");
+    return fail(MCGPU_ERR_HIP);
+  }
+  *out = b;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_destroy(mcgpu_box* b) {
+  free_box(b);
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_set_precision(mcgpu_box* b, int precision) {
+  if (!b || (precision != MCGPU_FP64 && precision != MCGPU_MIXED)) {
+    set_error("mcgpu_box_set_precision: bad argument");
+    return MCGPU_ERR_INVALID;
+  }
+  b->precision = precision;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_enable_timing(mcgpu_box* b, int on) {
+  if (!b) return MCGPU_ERR_INVALID;
+  if (on && !b->ev0) {
+    MCGPU_HIP_TRY(hipEventCreate(&b->ev0));
+    MCGPU_HIP_TRY(hipEventCreate(&b->ev1));
+  } else if (!on && b->ev0) {
+    MCGPU_HIP_TRY(hipEventDestroy(b->ev0));
+    MCGPU_HIP_TRY(hipEventDestroy(b->ev1));
+    b->ev0 = b->ev1 = nullptr;
+  }
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_last_kernel_ms(mcgpu_box* b, double* ms) {
+  if (!b || !ms) return MCGPU_ERR_INVALID;
+  *ms = b->last_ms;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_neighbor_rebuild(mcgpu_box* b, int kind, double skin) {
+  if (!b) return MCGPU_ERR_INVALID;
+  if (kind != MCGPU_NEIGHBOR_BRUTE && kind != MCGPU_NEIGHBOR_CELL &&
+      kind != MCGPU_NEIGHBOR_VERLET) {
+    set_error("mcgpu_neighbor_rebuild: unknown kind");
+    return MCGPU_ERR_INVALID;
+  }
+  if (kind == MCGPU_NEIGHBOR_VERLET && !(skin > 0)) {
+    set_error("mcgpu_neighbor_rebuild: Verlet skin must be > 0");
+    return MCGPU_ERR_INVALID;
+  }
+  b->bv.mode = kind;
+  b->skin = skin;
+  int rc = rebuild_neighbors(b);
+  if (rc) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_upload_coords(mcgpu_box* b, const double* x,
+                                   const double* y, const double* z) {
+  if (!b || !x || !y || !z) return MCGPU_ERR_INVALID;
+  const size_t n = b->bv.n_site;
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.x, x, 8 * n, hipMemcpyHostToDevice, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.y, y, 8 * n, hipMemcpyHostToDevice, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.z, z, 8 * n, hipMemcpyHostToDevice, b->stream));
+  int rc = refresh_primary(b);
+  if (rc) return rc;
+  rc = rebuild_neighbors(b);
+  if (rc) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_download_coords(mcgpu_box* b, double* x, double* y,
+                                     double* z) {
+  if (!b || !x || !y || !z) return MCGPU_ERR_INVALID;
+  const size_t n = b->bv.n_site;
+  MCGPU_HIP_TRY(hipMemcpyAsync(x, b->bv.x, 8 * n, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(y, b->bv.y, 8 * n, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(z, b->bv.z, 8 * n, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_set_molecule_coords(mcgpu_box* b, int32_t mol,
+                                         const double* xyz) {
+  int rc = check_mol(b, mol);
+  if (rc) return rc;
+  if (!xyz) return MCGPU_ERR_INVALID;
+  MoveArgs a{};
+  a.mol = mol;
+  if ((rc = fill_coords(b, mol, xyz, a)) != MCGPU_OK) return rc;
+  hipLaunchKernelGGL(k_set_mol, dim3(1), dim3(64), 0, b->stream, b->bv, a);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_get_molecule_coords(mcgpu_box* b, int32_t mol,
+                                         double* xyz) {
+  int rc = check_mol(b, mol);
+  if (rc) return rc;
+  if (!xyz) return MCGPU_ERR_INVALID;
+  const int len = b->h_mol_len[mol], s = b->h_mol_start[mol];
+  MCGPU_HIP_TRY(hipMemcpyAsync(xyz, b->bv.x + s, 8 * len, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(xyz + len, b->bv.y + s, 8 * len, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(xyz + 2 * len, b->bv.z + s, 8 * len, hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_molecule_energy(mcgpu_box* b, int32_t mol,
+                                     int32_t start_mol, double* energy) {
+  int rc = check_mol(b, mol);
+  if (rc) return rc;
+  if (!energy) return MCGPU_ERR_INVALID;
+  if (start_mol < 0 || start_mol > b->bv.n_mol) {
+    set_error("start_mol out of range");
+    return MCGPU_ERR_RANGE;
+  }
+  MoveArgs a{};
+  a.mol = mol;
+  a.start_mol = start_mol;
+  if ((rc = launch_move(b, a)) != MCGPU_OK) return rc;
+  *energy = b->h_out->e_old;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_trial_move(mcgpu_box* b, int32_t mol,
+                                const double* new_xyz, double u,
+                                mcgpu_move_result* res) {
+  int rc = check_mol(b, mol);
+  if (rc) return rc;
+  if (!new_xyz || !res) return MCGPU_ERR_INVALID;
+  MoveArgs a{};
+  a.mol = mol;
+  a.start_mol = 0;
+  a.has_new = 1;
+  a.metropolis = 1;
+  a.u = u;
+  if ((rc = fill_coords(b, mol, new_xyz, a)) != MCGPU_OK) return rc;
+  if ((rc = launch_move(b, a)) != MCGPU_OK) return rc;
+  res->old_energy = b->h_out->e_old;
+  res->new_energy = b->h_out->e_new;
+  res->delta = b->h_out->de;
+  res->accepted = b->h_out->accepted;
+  res->reserved = 0;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_system_energy(mcgpu_box* b, double* energy) {
+  if (!b || !energy) return MCGPU_ERR_INVALID;
+  const int n = b->bv.n_mol;
+  const int grid = std::min(n, 8192);
+  if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  if (b->precision == MCGPU_MIXED)
+    hipLaunchKernelGGL((k_system<64, true>), dim3(grid), dim3(64), 0, b->stream,
+                       b->bv, b->d_partial, b->d_pairs);
+  else
+    hipLaunchKernelGGL((k_system<64, false>), dim3(grid), dim3(64), 0, b->stream,
+                       b->bv, b->d_partial, b->d_pairs);
+  MCGPU_HIP_TRY(hipGetLastError());
+  if (b->ev1) MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, b->stream, b->d_partial,
+                     b->d_pairs, n, b->d_total);
+  MCGPU_HIP_TRY(hipGetLastError());
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->ev0) MCGPU_HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  *energy = b->h_total[0];
+  b->last_pairs = (int64_t)b->h_total[1];
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_system_pair_count(mcgpu_box* b, int64_t* pairs) {
+  if (!b || !pairs) return MCGPU_ERR_INVALID;
+  *pairs = b->last_pairs;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_volume_scale(mcgpu_box* b, double factor) {
+  if (!b || !(factor > 0) || !std::isfinite(factor)) {
+    set_error("mcgpu_volume_scale: factor must be finite and > 0");
+    return MCGPU_ERR_INVALID;
+  }
+  const int n = b->bv.n_mol;
+  hipLaunchKernelGGL(k_volume_scale, dim3((n + 255) / 256), dim3(256), 0,
+                     b->stream, b->bv, factor);
+  MCGPU_HIP_TRY(hipGetLastError());
+  for (int k = 0; k < 3; k++) b->bv.L[k] *= factor;
+  int rc = refresh_primary(b);
+  if (rc) return rc;
+  if ((rc = rebuild_neighbors(b)) != MCGPU_OK) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_size(mcgpu_box* b, double* edges3) {
+  if (!b || !edges3) return MCGPU_ERR_INVALID;
+  for (int k = 0; k < 3; k++) edges3[k] = b->bv.L[k];
+  return MCGPU_OK;
+}

@@ -1,0 +1,61 @@
+// Host-side view of a device-resident box (shared by the single-box and
+// batched translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+#include "mcgpu_energy.h"
+
+namespace mcgpu {
+
+void set_error(const std::string& msg);
+std::string validate_desc(const mcgpu_box_desc& d);
+
+#define MCGPU_HIP_TRY(call)                                                  \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::mcgpu::set_error(std::string(#call) + " failed: " +                  \
+                         hipGetErrorString(e_));                             \
+      return MCGPU_ERR_HIP;                                                  \
+    }                                                                        \
+  } while (0)
+
+// Result of a fused trial move / molecule energy launch.
+struct MoveOut {
+  double e_old;
+  double e_new;
+  double de;
+  int32_t accepted;
+  int32_t source;  // neighbour source used (MCGPU_NB_*)
+};
+
+// Molecule signatures -> site parameter table, plus per-molecule base index.
+struct ParamTable {
+  std::vector<SiteParam64> p64;
+  std::vector<SiteParam32> p32;
+  std::vector<int32_t> mol_param;
+  double spread = 0.0;  // max distance of a primary from the first primary
+  int32_t max_len = 0;
+};
+ParamTable build_param_table(const mcgpu_box_desc& d);
+
+// Cell grid geometry for a box: cells of width >= rc + 2 * spread.
+void cell_geometry(const double L[3], double cutoff, double spread, int n[3],
+                   double inv_w[3]);
+
+int ensure_device();
+
+// Launch helpers shared by the single-box and batched paths (stream-ordered,
+// no synchronisation).
+int launch_refresh_primary(const BoxView& bv, int n_pidx, hipStream_t s);
+int launch_cells_build(const BoxView& bv, hipStream_t s);
+int launch_system_energy(const BoxView& bv, int precision, double* d_partial,
+                         double* d_pairs, double* d_out2, hipStream_t s);
+int launch_volume_scale(const BoxView& bv, double f, hipStream_t s);
+
+}  // namespace mcgpu

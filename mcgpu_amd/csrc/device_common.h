@@ -1,0 +1,243 @@
+// Device-side building blocks of the energy path for gfx950 (CDNA4, wave64).
+//
+// Data layout in HBM (one box):
+//   x, y, z          [n_site]  FP64 master coordinates (SoA, coalesced)
+//   px, py, pz       [n_pidx]  FP64 mirror of the primary sites, so the
+//                              candidate scan reads 24 contiguous bytes per
+//                              molecule instead of gathering from x/y/z
+//   mol_start/len, mol_pidx_start/count, mol_param   [n_mol] int32
+//   p64 / p32        per distinct molecule signature, per site:
+//                    {sqrt(sigma), sqrt(eps), q, real}; sigma_ab =
+//                    sqrt(sigma_a) sqrt(sigma_b), eps_ab likewise, which is
+//                    the reference's geometric mixing (SimulationStep.cpp:
+//                    369-375) without a per-pair square root
+//   cell lists       count[ncell], slots[ncell*cap], mol_cell/mol_slot[n_mol]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rng31.h"
+
+namespace mcgpu {
+
+constexpr int kMaxSites = 32;        // MCGPU_MAX_MOL_SITES
+constexpr double kCoulombK = 332.06; // SimulationStep.cpp:364
+
+struct SiteParam64 {
+  double ss, se, q, real;
+};
+struct SiteParam32 {
+  float ss, se, q, real;
+};
+
+struct CellGrid {
+  int32_t n[3];       // cells per dimension (>= 1)
+  int32_t cap;        // slots per cell
+  double inv_w[3];    // 1 / cell width
+  int32_t* count;     // [ncell]
+  int32_t* slots;     // [ncell * cap]
+  int32_t* mol_cell;  // [n_mol]
+  int32_t* mol_slot;  // [n_mol]
+  int32_t* overflow;  // [1]: nonzero -> lists incomplete, scan all molecules
+};
+
+struct VerletList {
+  int32_t cap;        // entries per molecule
+  double skin_half2;  // (skin/2)^2
+  int32_t* count;     // [n_mol]
+  int32_t* list;      // [n_mol * cap]
+  double* ref;        // [3*n_mol] first-primary position at build
+  int32_t* stale;     // [1]: nonzero -> some molecule moved > skin/2
+};
+
+#define MCGPU_NB_BRUTE 0
+#define MCGPU_NB_CELL 1
+#define MCGPU_NB_VERLET 2
+
+struct BoxView {
+  double* x;
+  double* y;
+  double* z;
+  double* px;
+  double* py;
+  double* pz;
+  const int32_t* mol_start;
+  const int32_t* mol_len;
+  const int32_t* mol_pidx_start;
+  const int32_t* mol_pidx_count;
+  const int32_t* mol_param;
+  const int32_t* pidx;
+  const SiteParam64* p64;
+  const SiteParam32* p32;
+  int32_t n_mol;
+  int32_t n_site;
+  int32_t max_len;
+  int32_t mode;  // MCGPU_NB_*
+  double L[3];
+  double cutoff2;
+  double kT;
+  CellGrid cells;
+  VerletList verlet;
+};
+
+// ---------------------------------------------------------------- math
+
+// SimulationStep.cpp:377-385: one wrap per dimension.
+__device__ __forceinline__ double wrap1(double d, double L) {
+  d += (d < -0.5 * L) ? L : 0.0;
+  d -= (d > 0.5 * L) ? L : 0.0;
+  return d;
+}
+
+// FP64 LJ + Coulomb of one site pair from the wrapped displacement
+// (SimulationStep.cpp:344-367; BruteForceStep.cpp:64-73).
+__device__ __forceinline__ double pair_energy64(double dx, double dy, double dz,
+                                                const SiteParam64& a,
+                                                const SiteParam64& b) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 == 0.0 || a.real == 0.0 || b.real == 0.0) return 0.0;
+  const double inv_r2 = 1.0 / r2;
+  const double sig = a.ss * b.ss;
+  const double s2 = sig * sig * inv_r2;
+  const double s6 = s2 * s2 * s2;
+  const double lj = 4.0 * (a.se * b.se) * (s6 * s6 - s6);
+  const double coul = (a.q * b.q * kCoulombK) * sqrt(inv_r2);
+  return lj + coul;
+}
+
+// Mixed precision: displacement wrapped in FP64, pair terms in FP32, the
+// result handed back for FP64 accumulation.  Near-contact LJ pairs
+// (s^6 > 4, r < 0.79 sigma) are re-evaluated in FP64: there the FP32
+// rounding of s^12 (~20 ulp) would reach 1e-5..1e-3 kcal/mol per pair.
+constexpr float kNearContactS6 = 4.0f;
+
+__device__ __forceinline__ double pair_energy32(double dx, double dy, double dz,
+                                                const SiteParam32& a,
+                                                const SiteParam32& b,
+                                                const SiteParam64& a64,
+                                                const SiteParam64* b64) {
+  const double r2d = dx * dx + dy * dy + dz * dz;
+  const float r2 = (float)r2d;
+  if (r2 == 0.0f || a.real == 0.0f || b.real == 0.0f) return 0.0;
+  float rinv = __builtin_amdgcn_rsqf(r2);
+  rinv = rinv * (1.5f - 0.5f * r2 * rinv * rinv);  // one Newton step
+  const float inv_r2 = rinv * rinv;
+  const float sig = a.ss * b.ss;
+  const float s2 = sig * sig * inv_r2;
+  const float s6 = s2 * s2 * s2;
+  const float eps = a.se * b.se;
+  const float coul = (a.q * b.q * 332.06f) * rinv;
+  double lj;
+  if (s6 > kNearContactS6 && eps != 0.0f) {
+    const SiteParam64 bb = *b64;
+    const double sd = a64.ss * bb.ss;
+    const double s2d = sd * sd / r2d;
+    const double s6d = s2d * s2d * s2d;
+    lj = 4.0 * (a64.se * bb.se) * (s6d * s6d - s6d);
+  } else {
+    lj = (double)(4.0f * eps * (s6 * s6 - s6));
+  }
+  return lj + (double)coul;
+}
+
+// ------------------------------------------------------- wave / block ops
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block sum in a fixed order (deterministic for a given NT).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  if (lane_id() == 0) lds[wave_id()] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive prefix of a per-thread flag across the block, in thread order.
+template <int NT>
+__device__ __forceinline__ int block_scan_flag(bool f, int* lds, int& total) {
+  const unsigned long long m = __ballot(f);
+  const int lane = lane_id();
+  const int below = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) lds[wave_id()] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    const int c = lds[w];
+    off += (w < wave_id()) ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return off + below;
+}
+
+// ------------------------------------------------------------ cell grid
+
+__device__ __forceinline__ int cell_coord(double p, double inv_w, int n) {
+  int c = (int)floor(p * inv_w);
+  c = c < 0 ? 0 : c;
+  return c >= n ? n - 1 : c;
+}
+
+__device__ __forceinline__ int cell_index(const CellGrid& g, int cx, int cy,
+                                          int cz) {
+  return (cx * g.n[1] + cy) * g.n[2] + cz;
+}
+
+// A range of cells per dimension covering the neighbourhoods of up to two
+// positions (old and proposed); never wraps onto itself.
+struct CellRange {
+  int lo[3];
+  int cnt[3];
+  __device__ int total() const { return cnt[0] * cnt[1] * cnt[2]; }
+  __device__ int cell(const CellGrid& g, int k) const {
+    const int c2 = k % cnt[2];
+    const int t = k / cnt[2];
+    const int c1 = t % cnt[1];
+    const int c0 = t / cnt[1];
+    int a = lo[0] + c0, b = lo[1] + c1, c = lo[2] + c2;
+    a += (a < 0) ? g.n[0] : 0;
+    a -= (a >= g.n[0]) ? g.n[0] : 0;
+    b += (b < 0) ? g.n[1] : 0;
+    b -= (b >= g.n[1]) ? g.n[1] : 0;
+    c += (c < 0) ? g.n[2] : 0;
+    c -= (c >= g.n[2]) ? g.n[2] : 0;
+    return cell_index(g, a, b, c);
+  }
+};
+
+__device__ __forceinline__ CellRange make_range(const CellGrid& g,
+                                                const int co[3],
+                                                const int cn[3]) {
+  CellRange r;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int lo = (co[d] < cn[d] ? co[d] : cn[d]) - 1;
+    const int hi = (co[d] > cn[d] ? co[d] : cn[d]) + 1;
+    const int c = hi - lo + 1;
+    if (c >= g.n[d]) {
+      r.lo[d] = 0;
+      r.cnt[d] = g.n[d];
+    } else {
+      r.lo[d] = lo;
+      r.cnt[d] = c;
+    }
+  }
+  return r;
+}
+
+}  // namespace mcgpu

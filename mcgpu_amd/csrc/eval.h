@@ -1,0 +1,298 @@
+// Work-group level evaluation of one molecule's intermolecular energy at its
+// current and (optionally) proposed position, in one pass over candidates.
+//
+// Phase 1 (candidate scan): every lane takes one candidate molecule j from
+// the neighbour source (all molecules, the cell range around the old and new
+// cells, or the Verlet list), tests the primary-site cutoff for the old and
+// new position (SimulationStep.cpp:317-330), and the hits are compacted into
+// LDS in candidate order by a ballot/popcount block scan.
+// Phase 2 (pair sum): lanes take (hit, site of j) items and loop over the
+// moving molecule's sites held in LDS, accumulating FP64 sums
+// (BruteForceStep.cpp:48-79).  A fixed-order block reduction follows, so the
+// result is deterministic for a given work-group size.
+#pragma once
+
+#include "device_common.h"
+
+namespace mcgpu {
+
+template <int NT>
+struct EvalShared {
+  static constexpr int kCap = NT * 4 < 256 ? 256 : NT * 4;
+  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
+  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
+  SiteParam64 q64[kMaxSites];
+  SiteParam32 q32[kMaxSites];
+  int32_t pofs[kMaxSites];  // primary offsets within the molecule
+  int32_t cand_j[kCap];
+  int32_t cand_f[kCap];  // bit0: in range at old position, bit1: at new
+  double red[NT / 64];
+  int32_t scan[NT / 64];
+  int32_t npri;
+  int32_t len;
+  int32_t nreal;
+  int32_t flag;
+};
+
+// Loads molecule `mol`'s current sites (and parameters) into LDS.
+template <int NT>
+__device__ __forceinline__ void load_molecule(const BoxView& bv, int mol,
+                                              EvalShared<NT>& sh) {
+  const int t = threadIdx.x;
+  const int start = bv.mol_start[mol];
+  const int len = bv.mol_len[mol];
+  const int pbase = bv.mol_param[mol];
+  for (int k = t; k < len; k += NT) {
+    sh.ox[k] = bv.x[start + k];
+    sh.oy[k] = bv.y[start + k];
+    sh.oz[k] = bv.z[start + k];
+    sh.q64[k] = bv.p64[pbase + k];
+    sh.q32[k] = bv.p32[pbase + k];
+  }
+  if (t == 0) {
+    const int ps = bv.mol_pidx_start[mol], pc = bv.mol_pidx_count[mol];
+    for (int k = 0; k < pc; k++) sh.pofs[k] = bv.pidx[ps + k] - start;
+    sh.npri = pc;
+    sh.len = len;
+    int nr = 0;
+    for (int k = 0; k < len; k++) nr += bv.p64[pbase + k].real != 0.0;
+    sh.nreal = nr;
+  }
+}
+
+__device__ __forceinline__ bool primary_in_range(const BoxView& bv, int j,
+                                                 const double* X,
+                                                 const double* Y,
+                                                 const double* Z,
+                                                 const int32_t* pofs,
+                                                 int npri) {
+  const int ps = bv.mol_pidx_start[j], pc = bv.mol_pidx_count[j];
+  bool in = false;
+  for (int q = 0; q < pc; q++) {
+    const double qx = bv.px[ps + q], qy = bv.py[ps + q], qz = bv.pz[ps + q];
+    for (int p = 0; p < npri; p++) {
+      const int o = pofs[p];
+      const double dx = wrap1(qx - X[o], bv.L[0]);
+      const double dy = wrap1(qy - Y[o], bv.L[1]);
+      const double dz = wrap1(qz - Z[o], bv.L[2]);
+      in |= (dx * dx + dy * dy + dz * dz) <= bv.cutoff2;
+    }
+  }
+  return in;
+}
+
+// Sums the pair energies of the compacted hits into e_old / e_new.
+template <int NT, bool MIXED>
+__device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
+                                           int nhits, double& e_old,
+                                           double& e_new,
+                                           unsigned long long& npairs) {
+  const int M = bv.max_len;
+  const int len = sh.len;
+  for (int item = threadIdx.x; item < nhits * M; item += NT) {
+    const int k = item / M;
+    const int b = item - k * M;
+    const int j = sh.cand_j[k];
+    if (b >= bv.mol_len[j]) continue;
+    const int f = sh.cand_f[k];
+    const int site = bv.mol_start[j] + b;
+    const int pi = bv.mol_param[j] + b;
+    const double xb = bv.x[site], yb = bv.y[site], zb = bv.z[site];
+    if (MIXED) {
+      const SiteParam32 pb = bv.p32[pi];
+      if (pb.real == 0.0f) continue;
+      if (f & 1) {
+        for (int a = 0; a < len; a++)
+          e_old += pair_energy32(wrap1(xb - sh.ox[a], bv.L[0]),
+                                 wrap1(yb - sh.oy[a], bv.L[1]),
+                                 wrap1(zb - sh.oz[a], bv.L[2]), sh.q32[a], pb,
+                                 sh.q64[a], bv.p64 + pi);
+        npairs += sh.nreal;
+      }
+      if (f & 2) {
+        for (int a = 0; a < len; a++)
+          e_new += pair_energy32(wrap1(xb - sh.nx[a], bv.L[0]),
+                                 wrap1(yb - sh.ny[a], bv.L[1]),
+                                 wrap1(zb - sh.nz[a], bv.L[2]), sh.q32[a], pb,
+                                 sh.q64[a], bv.p64 + pi);
+      }
+    } else {
+      const SiteParam64 pb = bv.p64[pi];
+      if (pb.real == 0.0) continue;
+      if (f & 1) {
+        for (int a = 0; a < len; a++)
+          e_old += pair_energy64(wrap1(xb - sh.ox[a], bv.L[0]),
+                                 wrap1(yb - sh.oy[a], bv.L[1]),
+                                 wrap1(zb - sh.oz[a], bv.L[2]), sh.q64[a], pb);
+        npairs += sh.nreal;
+      }
+      if (f & 2) {
+        for (int a = 0; a < len; a++)
+          e_new += pair_energy64(wrap1(xb - sh.nx[a], bv.L[0]),
+                                 wrap1(yb - sh.ny[a], bv.L[1]),
+                                 wrap1(zb - sh.nz[a], bv.L[2]), sh.q64[a], pb);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Neighbour source actually usable for this evaluation (block-uniform).
+__device__ __forceinline__ int pick_source(const BoxView& bv, const double* ox,
+                                           const double* oy, const double* oz,
+                                           const double* nx, const double* ny,
+                                           const double* nz, int p0, int mol,
+                                           bool has_new) {
+  if (bv.mode == MCGPU_NB_CELL && bv.cells.overflow[0] == 0) return MCGPU_NB_CELL;
+  if (bv.mode == MCGPU_NB_VERLET && bv.verlet.stale[0] == 0 &&
+      bv.verlet.count[mol] <= bv.verlet.cap) {
+    const double* r = bv.verlet.ref + 3 * mol;
+    if (has_new) {
+      const double dx = wrap1(nx[p0] - r[0], bv.L[0]);
+      const double dy = wrap1(ny[p0] - r[1], bv.L[1]);
+      const double dz = wrap1(nz[p0] - r[2], bv.L[2]);
+      if (dx * dx + dy * dy + dz * dz > bv.verlet.skin_half2) return MCGPU_NB_BRUTE;
+    }
+    return MCGPU_NB_VERLET;
+  }
+  return MCGPU_NB_BRUTE;
+}
+
+// Evaluates E_mol over candidates j >= jlo (j != mol) at the LDS-held old
+// position, and at the proposed position when has_new.  Must be called by
+// all NT threads after load_molecule (+ proposed coords) and a barrier.
+// Results are block-uniform.
+template <int NT, bool MIXED>
+__device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
+                              bool has_new, EvalShared<NT>& sh, double& e_old,
+                              double& e_new, unsigned long long* pair_count) {
+  const int t = threadIdx.x;
+  const int p0 = sh.pofs[0];
+  const int src = pick_source(bv, sh.ox, sh.oy, sh.oz, sh.nx, sh.ny, sh.nz, p0,
+                              mol, has_new);
+  CellRange range{};
+  int nitems;
+  const int cap = bv.cells.cap;
+  if (src == MCGPU_NB_CELL) {
+    int co[3], cn[3];
+    co[0] = cell_coord(sh.ox[p0], bv.cells.inv_w[0], bv.cells.n[0]);
+    co[1] = cell_coord(sh.oy[p0], bv.cells.inv_w[1], bv.cells.n[1]);
+    co[2] = cell_coord(sh.oz[p0], bv.cells.inv_w[2], bv.cells.n[2]);
+    if (has_new) {
+      cn[0] = cell_coord(sh.nx[p0], bv.cells.inv_w[0], bv.cells.n[0]);
+      cn[1] = cell_coord(sh.ny[p0], bv.cells.inv_w[1], bv.cells.n[1]);
+      cn[2] = cell_coord(sh.nz[p0], bv.cells.inv_w[2], bv.cells.n[2]);
+    } else {
+      cn[0] = co[0];
+      cn[1] = co[1];
+      cn[2] = co[2];
+    }
+    range = make_range(bv.cells, co, cn);
+    nitems = range.total() * cap;
+  } else if (src == MCGPU_NB_VERLET) {
+    nitems = bv.verlet.count[mol];
+  } else {
+    nitems = bv.n_mol - jlo;
+  }
+
+  double eo = 0.0, en = 0.0;
+  unsigned long long np = 0;
+  int nhits = 0;
+  for (int base = 0; base < nitems; base += NT) {
+    const int item = base + t;
+    int j = -1;
+    if (item < nitems) {
+      if (src == MCGPU_NB_CELL) {
+        const int k = item / cap;
+        const int s = item - k * cap;
+        const int c = range.cell(bv.cells, k);
+        j = (s < bv.cells.count[c]) ? bv.cells.slots[c * cap + s] : -1;
+      } else if (src == MCGPU_NB_VERLET) {
+        j = bv.verlet.list[mol * bv.verlet.cap + item];
+      } else {
+        j = jlo + item;
+      }
+    }
+    int f = 0;
+    if (j >= jlo && j != mol) {
+      f = primary_in_range(bv, j, sh.ox, sh.oy, sh.oz, sh.pofs, sh.npri) ? 1 : 0;
+      if (has_new)
+        f |= primary_in_range(bv, j, sh.nx, sh.ny, sh.nz, sh.pofs, sh.npri) ? 2 : 0;
+    }
+    int total;
+    const int pos = block_scan_flag<NT>(f != 0, sh.scan, total);
+    if (nhits + total > EvalShared<NT>::kCap) {
+      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np);
+      nhits = 0;
+    }
+    if (f) {
+      sh.cand_j[nhits + pos] = j;
+      sh.cand_f[nhits + pos] = f;
+    }
+    nhits += total;
+    __syncthreads();
+  }
+  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np);
+  e_old = block_sum<NT>(eo, sh.red);
+  e_new = has_new ? block_sum<NT>(en, sh.red) : 0.0;
+  if (pair_count) {
+    const double c = block_sum<NT>((double)np, sh.red);
+    if (t == 0) *pair_count = (unsigned long long)c;
+  }
+}
+
+// Writes the LDS-held proposed coordinates of `mol` into the box and keeps
+// the primary mirror, cell lists and Verlet bookkeeping consistent.
+template <int NT>
+__device__ void commit_molecule(const BoxView& bv, int mol,
+                                const EvalShared<NT>& sh) {
+  const int t = threadIdx.x;
+  const int start = bv.mol_start[mol];
+  for (int k = t; k < sh.len; k += NT) {
+    bv.x[start + k] = sh.nx[k];
+    bv.y[start + k] = sh.ny[k];
+    bv.z[start + k] = sh.nz[k];
+  }
+  if (t == 0) {
+    const int ps = bv.mol_pidx_start[mol];
+    for (int k = 0; k < sh.npri; k++) {
+      bv.px[ps + k] = sh.nx[sh.pofs[k]];
+      bv.py[ps + k] = sh.ny[sh.pofs[k]];
+      bv.pz[ps + k] = sh.nz[sh.pofs[k]];
+    }
+    const int p0 = sh.pofs[0];
+    if (bv.mode == MCGPU_NB_CELL && bv.cells.overflow[0] == 0) {
+      const CellGrid& g = bv.cells;
+      const int cnew = cell_index(g, cell_coord(sh.nx[p0], g.inv_w[0], g.n[0]),
+                                  cell_coord(sh.ny[p0], g.inv_w[1], g.n[1]),
+                                  cell_coord(sh.nz[p0], g.inv_w[2], g.n[2]));
+      const int cold = g.mol_cell[mol];
+      if (cnew != cold) {
+        const int slot = g.mol_slot[mol];
+        const int last = g.count[cold] - 1;
+        const int lm = g.slots[cold * g.cap + last];
+        g.slots[cold * g.cap + slot] = lm;
+        g.mol_slot[lm] = slot;
+        g.count[cold] = last;
+        const int n = g.count[cnew];
+        if (n < g.cap) {
+          g.slots[cnew * g.cap + n] = mol;
+          g.mol_slot[mol] = n;
+          g.count[cnew] = n + 1;
+          g.mol_cell[mol] = cnew;
+        } else {
+          g.overflow[0] = 1;  // lists incomplete from here: scans fall back
+        }
+      }
+    }
+    if (bv.mode == MCGPU_NB_VERLET) {
+      const double* r = bv.verlet.ref + 3 * mol;
+      const double dx = wrap1(sh.nx[p0] - r[0], bv.L[0]);
+      const double dy = wrap1(sh.ny[p0] - r[1], bv.L[1]);
+      const double dz = wrap1(sh.nz[p0] - r[2], bv.L[2]);
+      if (dx * dx + dy * dy + dz * dz > bv.verlet.skin_half2) bv.verlet.stale[0] = 1;
+    }
+  }
+}
+
+}  // namespace mcgpu

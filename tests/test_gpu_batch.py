@@ -1,0 +1,135 @@
+"""GPU parity of the batched persistent path: device RNG + device Metropolis
+per box must reproduce the reference's (and the oracle's) trial-move
+sequences, alone and mixed with other boxes; mixed precision stays within
+its stated bound; NPT volume moves keep the tracked energy consistent."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mcgpu_amd as m
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+DE_TOL = 1e-9
+
+
+def load(name):
+    return m.HostBox.from_state(str(GOLDEN / f"{name}.state"))
+
+
+def de_ok(got, want):
+    return np.all(np.abs(got - want) <= DE_TOL + 1e-14 * np.abs(want))
+
+
+def run_all(batch, slots, total, chunk):
+    done = 0
+    while done < total:
+        batch.run(min(chunk, total - done))
+        done += chunk
+    return [batch.stats(s) for s in slots]
+
+
+def test_batch_fp64_reproduces_reference_trace():
+    ref = np.load(GOLDEN / "water256.ref.npz")
+    steps = int(ref["steps"])
+    b = m.Batch(4)
+    s = b.add(load("water256"), steps=steps, seed=1, record_trace=True)
+    (st,) = run_all(b, [s], steps, 2500)
+    tr = b.trace(s)
+    assert st.steps_done == steps and st.done
+    assert np.array_equal(tr.mol, ref["mol"])
+    assert np.array_equal(tr.accepted, ref["accepted"])
+    assert np.max(np.abs(tr.de - ref["de"])) <= DE_TOL
+    assert st.start_energy == pytest.approx(float(ref["e0"]), rel=1e-12)
+    assert st.energy == pytest.approx(float(ref["final"]), rel=1e-12, abs=1e-7)
+    assert st.accepted == int(ref["n_accepted"])
+
+
+def test_batch_mix_each_box_equals_its_own_oracle_run():
+    """Boxes of different kinds, sizes and seeds in one launch; each one's
+    trace must equal the oracle run of that box alone."""
+    steps = 1500
+    boxes = [("water256", 1), ("water256", 7), ("bicyclic200", 3),
+             ("mixture300", 11), ("water128_mp", 5)]
+    b = m.Batch(8)
+    slots, want = [], []
+    for name, seed in boxes:
+        hb = load(name)
+        ob = O.OracleBox(hb.arrays())
+        _, _, _, t = ob.run_nvt(seed, steps, use_libc=False)
+        want.append(t)
+        slots.append(b.add(hb, steps=steps, seed=seed, record_trace=True))
+    run_all(b, slots, steps, 500)
+    for s, w, (name, seed) in zip(slots, want, boxes):
+        tr = b.trace(s)
+        assert np.array_equal(tr.mol, w["mol"]), name
+        assert np.array_equal(tr.accepted, w["accepted"]), name
+        assert de_ok(tr.de, w["de"]), name
+
+
+def test_batch_cells_match_brute_force_trace():
+    hb = m.workload("bicyclic4000", n_molecules=1500)
+    steps = 1500
+    ob = O.OracleBox(hb.arrays())
+    _, _, _, w = ob.run_nvt(hb.mc_seed, steps, use_libc=False)
+    b = m.Batch(2)
+    s = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+    run_all(b, [s], steps, 750)
+    tr = b.trace(s)
+    assert np.array_equal(tr.mol, w["mol"])
+    assert np.array_equal(tr.accepted, w["accepted"])
+    assert de_ok(tr.de, w["de"])
+
+
+def test_batch_mixed_precision_error():
+    ref = np.load(GOLDEN / "water256.ref.npz")
+    steps = int(ref["steps"])
+    b = m.Batch(2)
+    s = b.add(load("water256"), steps=steps, seed=1, precision=m.MIXED,
+              record_trace=True)
+    run_all(b, [s], steps, 5000)
+    tr = b.trace(s)
+    same = np.cumprod(tr.accepted == ref["accepted"]).astype(bool)
+    n = int(same.sum())
+    err = np.abs(tr.de[:n] - ref["de"][:n])
+    print(f"batch mixed water256: {n}/{steps} aligned, max {err.max():.3e}, "
+          f"p99 {np.quantile(err, 0.99):.3e}, median {np.median(err):.3e}")
+    assert n >= 2000
+    assert np.quantile(err, 0.99) <= 1e-4
+
+
+def test_batch_npt_energy_stays_consistent():
+    hb = load("mixture300")
+    steps = 2000
+    b = m.Batch(2)
+    s = b.add(hb, steps=steps, ensemble=m.NPT, volume_interval=250,
+              pressure=1.4584e-5, max_volume_delta=0.02, neighbor=m.NEIGHBOR_CELL)
+    (st,) = run_all(b, [s], steps, 400)
+    assert st.steps_done == steps and st.done
+    assert st.volume_moves == 7  # at 250, 500, ..., 1750 moves
+    x, y, z = b.download(s)
+    a = hb.arrays()
+    a.update(x=x, y=y, z=z, box=np.array(st.box[:]))
+    ob = O.OracleBox(a)
+    e = ob.system_energy() + ob.lrc_reference()
+    assert st.energy == pytest.approx(e, rel=1e-9, abs=1e-6)
+
+
+def test_batch_admission_by_footprint():
+    hb = load("water256")
+    import ctypes as C
+    d = hb.desc()
+    p = m.RunParams(1, 0, 0, 0, 1000, 0.0, 0.0, 0.0, 0, 0)
+    need = C.c_size_t()
+    m.check(m.lib().mcgpu_box_footprint(C.byref(d), C.byref(p), C.byref(need)))
+    b = m.Batch(8, hbm_budget=int(need.value * 2.5))
+    s1 = b.add(hb, steps=1000)
+    s2 = b.add(hb, steps=1000)
+    with pytest.raises(m.McgpuError, match="budget"):
+        b.add(hb, steps=1000)
+    b.remove(s1)
+    s3 = b.add(hb, steps=1000)
+    b.run(100)
+    assert b.stats(s3).steps_done == 100 and b.stats(s2).steps_done == 100

@@ -1,0 +1,169 @@
+"""GPU parity of the single-box path (C ABI -> HIP kernels) against the
+oracle and the reference's own per-move traces.
+
+Bar: FP64 energies within 1e-9 kcal/mol (relative 1e-12 for large sums)
+of the oracle; trial-move traces with identical molecule choices and
+accept/reject decisions and |dE - dE_ref| <= 1e-9 kcal/mol + 1e-14 |dE_ref|.
+The relative term only matters for trial positions that overlap another
+molecule (|dE| up to 1e12 kcal/mol in the generated 16-site boxes), where
+FP64 summation order alone moves the last bits; on the 256-water box every
+move meets the absolute 1e-9 bound.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mcgpu_amd as m
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+CASES = ["water256", "bicyclic200", "mixture300", "water128_mp"]
+DE_TOL = 1e-9
+
+
+def de_ok(got, want):
+    return np.all(np.abs(got - want) <= DE_TOL + 1e-14 * np.abs(want))
+
+
+def load(name):
+    return m.HostBox.from_state(str(GOLDEN / f"{name}.state"))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_molecule_energy_matches_oracle(name):
+    hb = load(name)
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    n = db.n_mol
+    rng = np.random.default_rng(0)
+    mols = list(range(min(n, 40))) + list(rng.integers(0, n, 40))
+    for i in mols:
+        for start in (0, int(i)):
+            want = ob.molecule_energy(i, start)
+            got = db.molecule_energy(i, start)
+            assert got == pytest.approx(want, rel=1e-12, abs=DE_TOL), (i, start)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_system_energy_matches_oracle(name):
+    hb = load(name)
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=DE_TOL)
+    assert db.system_pairs() > 0
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_plain_path_reproduces_reference_trace(name):
+    ref = np.load(GOLDEN / f"{name}.ref.npz")
+    steps = int(ref["steps"])
+    hb = load(name)
+    res, tr = m.plain_run(hb, steps, precision=m.FP64)
+    assert np.array_equal(tr.mol, ref["mol"])
+    assert np.array_equal(tr.accepted, ref["accepted"])
+    assert de_ok(tr.de, ref["de"])
+    if name == "water256":
+        assert np.max(np.abs(tr.de - ref["de"])) <= DE_TOL
+    assert res.start_energy == pytest.approx(float(ref["e0"]), rel=1e-12, abs=DE_TOL)
+    assert res.final_energy == pytest.approx(float(ref["final"]), rel=1e-12, abs=1e-7)
+    assert res.accepted == int(ref["n_accepted"])
+
+
+def test_strategy_calls_reproduce_reference_trace():
+    ref = np.load(GOLDEN / "water256.ref.npz")
+    steps = 3000
+    res, tr = m.strategy_run(load("water256"), steps)
+    assert np.array_equal(tr.mol, ref["mol"][:steps])
+    assert np.array_equal(tr.accepted, ref["accepted"][:steps])
+    assert np.max(np.abs(tr.de - ref["de"][:steps])) <= DE_TOL
+
+
+@pytest.mark.parametrize("kind,skin", [(m.NEIGHBOR_CELL, 0.0),
+                                       (m.NEIGHBOR_VERLET, 1.5)])
+def test_neighbor_structures_equal_brute_force(kind, skin):
+    hb = m.workload("bicyclic4000")
+    brute = m.DeviceBox(hb)
+    nb = m.DeviceBox(hb)
+    nb.neighbor(kind, skin)
+    rng = np.random.default_rng(1)
+    for i in rng.integers(0, 4000, 60):
+        assert nb.molecule_energy(int(i)) == pytest.approx(
+            brute.molecule_energy(int(i)), rel=1e-12, abs=1e-9)
+    assert nb.system_energy() == pytest.approx(brute.system_energy(), rel=1e-12)
+
+
+@pytest.mark.parametrize("kind,skin", [(m.NEIGHBOR_CELL, 0.0),
+                                       (m.NEIGHBOR_VERLET, 1.0)])
+def test_neighbor_paths_keep_trace_parity(kind, skin):
+    """Cell/Verlet lists under moves (incremental updates, stale lists)
+    against the oracle's brute-force trace on a box with >= 3 cells/dim."""
+    hb = m.workload("bicyclic4000", n_molecules=1500, density=1.10)
+    steps = 1500
+    ob = O.OracleBox(hb.arrays())
+    _, _, _, want = ob.run_nvt(hb.mc_seed, steps, use_libc=False)
+    res, tr = m.plain_run(hb, steps, precision=m.FP64, neighbor=kind)
+    assert np.array_equal(tr.mol, want["mol"])
+    assert np.array_equal(tr.accepted, want["accepted"])
+    assert de_ok(tr.de, want["de"])
+
+
+def test_mixed_precision_error_bound():
+    """FP32 pair terms, FP64 sums: per-move |dE - dE_fp64| over a trajectory
+    where both make the same decisions."""
+    ref = np.load(GOLDEN / "water256.ref.npz")
+    steps = 10000
+    res, tr = m.plain_run(load("water256"), steps, precision=m.MIXED)
+    same = np.cumprod(tr.accepted == ref["accepted"][:steps]).astype(bool)
+    n = int(same.sum())
+    err = np.abs(tr.de[:n] - ref["de"][:n])
+    print(f"mixed water256: {n} moves aligned, max |dE err| {err.max():.3e}, "
+          f"p99 {np.quantile(err, 0.99):.3e}")
+    assert n >= 1000
+    assert err.max() <= 1e-3
+    assert np.quantile(err, 0.99) <= 1e-4
+
+
+@pytest.mark.parametrize("f", [1.03, 0.98])
+def test_volume_scale_matches_oracle(f):
+    hb = load("mixture300")
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    db.neighbor(m.NEIGHBOR_CELL)
+    db.volume_scale(f)
+    ob.volume_scale(f)
+    assert np.allclose(db.size(), ob.L, rtol=0, atol=0)
+    x, y, z = db.download()
+    # the device may fuse p*f - p into one FMA: allow a few ulps
+    assert np.allclose(x, ob.a["x"], rtol=0, atol=1e-12)
+    assert np.allclose(z, ob.a["z"], rtol=0, atol=1e-12)
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12)
+
+
+def test_small_box_and_two_molecules():
+    # L < 2 rc after clamping; two molecules only
+    hb = m.HostBox.generate(m.WATER, 2, edge=10.0, cutoff=9.0, seed=3)
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    for i in range(2):
+        assert db.molecule_energy(i) == pytest.approx(ob.molecule_energy(i), rel=1e-12, abs=1e-12)
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=1e-12)
+
+
+def test_dummy_sites_and_errors():
+    hb = load("water256")
+    a = hb.arrays()
+    a["sigma"][1::3] = -1.0   # every first H becomes a dummy site
+    ob = O.OracleBox(a)
+    d = hb.desc()
+    import ctypes as C
+    keep = {k: np.ascontiguousarray(a[k]) for k in ("sigma",)}
+    d.sigma = keep["sigma"].ctypes.data_as(C.POINTER(C.c_double))
+    db = m.DeviceBox(d)
+    for i in range(0, 256, 17):
+        assert db.molecule_energy(i) == pytest.approx(ob.molecule_energy(i), rel=1e-12, abs=1e-9)
+    with pytest.raises(m.McgpuError):
+        db.molecule_energy(256)
+    with pytest.raises(m.McgpuError):
+        db.molecule_energy(0, 257)
