@@ -1,0 +1,319 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X energy path on the serving stream.
+
+Workload (the stream the metric is quoted on): seeded Poisson arrivals at
+200 jobs/s of Monte Carlo jobs drawn from two generators -- rigid TIP3P water
+boxes (density U[0.95, 1.05] g/cm3, T U[250, 400] K, rc 9 A) and rigid
+16-site planar bicyclic molecules (1.10 g/cm3, 298 K, rc 12 A) -- with
+128-16,384 molecules and 1,000-100,000 steps (both log-uniform), half NVT
+and half NPT (1 atm, a volume move every 500 steps), half FP64 and half
+FP32-pair arithmetic.  Molecule counts are drawn from 24 log-spaced classes
+so every class's box is generated once; each job has its own MC seed.
+
+A step is one job of the stream.  The timed region serves exactly K jobs
+(arrival of the first to completion of the last) through the continuous-
+batching server: value = trial moves / wall seconds.  Beside it:
+  - the plain FP64 GPU path (host RNG, one fused trial-move launch per
+    move) on a sample of the same stream's jobs, and the ratio;
+  - p50/p99 job latency (completion - arrival);
+  - the roofline of the dominant kernel (the persistent batch kernel):
+    algorithmic bytes from the kernel's own work counters (24 B per
+    candidate primary, 24 B per site loaded) / HIP-event launch time;
+  - the reference's own CPU path (oracle/_ref) timed on host cores on a
+    bounded sample of the same jobs (kind "reference"), or the oracle
+    restatement when the reference build is absent (kind "port").
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("MC trial moves/s, batched FP32/FP64-mixed vs plain FP64 on 1 GPU; "
+          "p50/p99 job latency")
+HBM_PEAK_GBS = 8000.0
+LAMBDA = 200.0
+N_CLASSES = 24
+
+
+def log_uniform(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), size))
+
+
+def make_stream(n_jobs: int, seed: int = 2024):
+    """The job stream: arrival times and per-job parameters (seeded)."""
+    import mcgpu_amd as m
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / LAMBDA, n_jobs))
+    arrivals -= arrivals[0]
+    classes = np.unique(np.round(np.exp(np.linspace(math.log(128), math.log(16384),
+                                                     N_CLASSES))).astype(int))
+    jobs = []
+    for k in range(n_jobs):
+        kind = m.WATER if rng.random() < 0.5 else m.BICYCLIC
+        n = int(log_uniform(rng, 128, 16384))
+        n = int(classes[np.argmin(np.abs(np.log(classes) - math.log(n)))])
+        steps = int(log_uniform(rng, 1e3, 1e5))
+        npt = rng.random() < 0.5
+        mixed = rng.random() < 0.5
+        jobs.append(dict(kind=kind, n=n, steps=steps, npt=npt, mixed=mixed,
+                         mc_seed=int(rng.integers(1, 2**31 - 1)),
+                         arrival=float(arrivals[k])))
+    return jobs
+
+
+class BoxPool:
+    """One generated box per (kind, size class), reused by every job of that
+    class (each job has its own MC seed)."""
+
+    def __init__(self, seed: int = 7):
+        self.boxes = {}
+        self.rng = np.random.default_rng(seed)
+
+    def get(self, kind: int, n: int):
+        import mcgpu_amd as m
+        key = (kind, n)
+        if key not in self.boxes:
+            if kind == m.WATER:
+                rho = float(self.rng.uniform(0.95, 1.05))
+                temp = float(self.rng.uniform(250, 400))
+                hb = m.HostBox.generate(m.WATER, n, density=rho, cutoff=9.0,
+                                        temperature=temp, max_translate=0.15,
+                                        max_rotate=15.0, seed=1000 + n)
+            else:
+                hb = m.HostBox.generate(m.BICYCLIC, n, density=1.10, cutoff=12.0,
+                                        temperature=298.0, max_translate=0.15,
+                                        max_rotate=15.0, seed=2000 + n)
+            self.boxes[key] = hb
+        return self.boxes[key]
+
+
+def neighbor_for(box) -> int:
+    """Cell lists once every dimension holds >= 3 cells of width rc."""
+    import mcgpu_amd as m
+    d = box.desc()
+    return m.NEIGHBOR_CELL if min(d.box) >= 3 * d.cutoff else m.NEIGHBOR_BRUTE
+
+
+def run_kwargs(job, box):
+    import mcgpu_amd as m
+    kw = dict(steps=job["steps"], seed=job["mc_seed"],
+              precision=m.MIXED if job["mixed"] else m.FP64,
+              neighbor=neighbor_for(box))
+    if job["npt"]:
+        kw.update(ensemble=m.NPT, volume_interval=500, pressure=1.4584e-5,
+                  max_volume_delta=0.01)
+    return kw
+
+
+def serve(jobs, pool, t_shift=True, sweeps=200, max_boxes=1024,
+          budget=64 << 30):
+    import mcgpu_amd as m
+    srv = m.Server(max_boxes=max_boxes, sweeps_per_tick=sweeps, hbm_budget=budget)
+    t0 = jobs[0]["arrival"] if t_shift else 0.0
+    for j in jobs:
+        box = pool.get(j["kind"], j["n"])
+        srv.submit(box, arrival_s=j["arrival"] - t0, **run_kwargs(j, box))
+    srv.run()
+    return srv.results(), srv.stats()
+
+
+def plain_sample(jobs, pool, budget_s: float, max_moves: int = 2000):
+    """Plain FP64 path on jobs of the stream in arrival order (each job run
+    for min(steps, max_moves) moves) until budget_s of loop time."""
+    import mcgpu_amd as m
+    moves, secs, used = 0, 0.0, 0
+    for j in jobs:
+        box = pool.get(j["kind"], j["n"])
+        with tempfile.TemporaryDirectory() as td:
+            p = Path(td) / "b.state"
+            box.write_state(str(p))
+            hb = m.HostBox.from_state(str(p))
+        steps = min(j["steps"], max_moves)
+        res, _ = m.plain_run(hb, steps, precision=m.FP64,
+                             neighbor=neighbor_for(hb), trace=False)
+        moves += steps
+        secs += res.loop_seconds
+        used += 1
+        if secs >= budget_s:
+            break
+    return moves / secs, used, moves
+
+
+def cpu_baseline(jobs, pool, budget_s: float):
+    """The reference's own CPU path (oracle/_ref/ref_trace, single thread,
+    brute force as in its -s mode) on the first jobs of the stream, each for
+    a bounded number of moves; falls back to the oracle restatement."""
+    import mcgpu_amd as m
+    ref = ROOT / "oracle" / "_ref" / "ref_trace"
+    moves, secs, used = 0, 0.0, 0
+    per_job = max(1.0, budget_s / 6)
+    kind = "reference" if ref.exists() else "port"
+    for j in jobs:
+        box = pool.get(j["kind"], j["n"])
+        # estimate moves that fit in per_job seconds (brute force ~ N^1)
+        steps = int(max(20, min(j["steps"], 4e6 / j["n"] * per_job / 2)))
+        with tempfile.TemporaryDirectory() as td:
+            p = Path(td) / "b.state"
+            box.write_state(str(p))
+            if kind == "reference":
+                out = subprocess.run([str(ref), str(p), str(steps), "-"],
+                                     capture_output=True, text=True, check=True).stdout
+                loop = float(re.search(r"Loop Seconds: ([0-9.eE+-]+)", out).group(1))
+            else:
+                from oracle import oracle as O
+                hb = m.HostBox.from_state(str(p))
+                ob = O.OracleBox(hb.arrays())
+                t = time.perf_counter()
+                ob.run_nvt(j["mc_seed"], steps, use_libc=True)
+                loop = time.perf_counter() - t
+        moves += steps
+        secs += loop
+        used += 1
+        if secs >= budget_s:
+            break
+    return dict(value=moves / secs, unit="moves/s", cores=1, kind=kind,
+                sample=(f"first {used} jobs of the stream, {moves} moves in total "
+                        f"(each job capped to ~{per_job:.0f} s), brute force, "
+                        f"single thread, {secs:.1f} s of loop time"))
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the batch kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
+    p = ROOT / "profiles" / "k_batch_serving_pmc.txt"
+    if not p.exists():
+        return None
+    m_ = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m_.group(1)) if m_ else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="jobs in the timed stream")
+    ap.add_argument("--warmup", type=int, default=32, help="warm-up jobs (untimed)")
+    ap.add_argument("--sweeps", type=int, default=200, help="moves per box per tick")
+    ap.add_argument("--plain-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (boxes would shard across GPUs with no exchange; see the "
+              "design notes)", file=sys.stderr)
+        return 2
+    import mcgpu_amd as m
+    if m.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+
+    t_setup = time.perf_counter()
+    stream = make_stream(args.warmup + args.steps)
+    warm, timed = stream[:args.warmup], stream[args.warmup:]
+    pool = BoxPool()
+    for j in stream:
+        pool.get(j["kind"], j["n"])
+    setup_s = time.perf_counter() - t_setup
+    print(f"# generated {len(pool.boxes)} boxes for {len(stream)} jobs in "
+          f"{setup_s:.1f} s", file=sys.stderr, flush=True)
+
+    # warm-up: same server path, saturated (all warm-up jobs at t=0)
+    wj = [dict(j, arrival=0.0, steps=min(j["steps"], 2000)) for j in warm]
+    serve(wj, pool, t_shift=False, sweeps=args.sweeps)
+
+    # timed region: exactly K jobs of the stream
+    t0 = time.perf_counter()
+    res, st = serve(timed, pool, sweeps=args.sweeps)
+    wall = time.perf_counter() - t0
+    bad = [r for r in res if r.status != 0]
+    if bad:
+        print(f"bench.py: {len(bad)} jobs failed", file=sys.stderr)
+        return 1
+    moves = int(st.total_moves)
+    value = moves / st.wall_s
+    lat = np.array([r.finish_s - r.arrival_s for r in res])
+    print(f"# serving: {len(res)} jobs, {moves} moves in {st.wall_s:.2f} s "
+          f"({value:.4g} moves/s), {st.ticks} ticks, peak resident "
+          f"{st.peak_resident}, p50 {np.median(lat):.3f} s p99 "
+          f"{np.quantile(lat, 0.99):.3f} s", file=sys.stderr, flush=True)
+
+    # saturated capacity: the same jobs, all arriving at t = 0
+    cap_jobs = [dict(j, arrival=0.0) for j in timed[: min(len(timed), 600)]]
+    _, cst = serve(cap_jobs, pool, t_shift=False, sweeps=args.sweeps)
+    capacity = cst.total_moves / cst.wall_s
+
+    # roofline of the batch kernel (timed serving run)
+    kernel_s = st.kernel_ms_total / 1e3
+    alg_bytes = 24.0 * (st.work_candidates + st.work_sites)
+    achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    per_launch = alg_bytes / max(st.ticks, 1)
+    traffic = pmc_traffic()
+
+    plain_rate, plain_jobs, plain_moves = plain_sample(timed, pool, args.plain_seconds)
+    print(f"# plain FP64 path: {plain_rate:.4g} moves/s over {plain_jobs} jobs",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(timed, pool, args.cpu_seconds)
+        cpu["cores_available"] = os.cpu_count()
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": len(timed),
+        "warmup": len(warm),
+        "ms_per_step": 1e3 * st.wall_s / len(timed),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 coordinates/sums + f32-or-f64 pair terms",
+        "data": "synthetic",
+        "config": {
+            "workload": ("serving stream: Poisson 200 jobs/s, water (c2 generator) "
+                         "and 16-site bicyclic (c1 generator) boxes, 128-16384 "
+                         "molecules, 1e3-1e5 steps, NVT/NPT, FP64/FP32-pair mixed"),
+            "jobs": len(timed), "sweeps_per_tick": args.sweeps,
+            "parallelism": "boxes in one persistent launch",
+        },
+        "p50_latency_s": float(np.median(lat)),
+        "p99_latency_s": float(np.quantile(lat, 0.99)),
+        "plain_fp64_moves_per_s": plain_rate,
+        "plain_sample": f"{plain_jobs} jobs of the stream, {plain_moves} moves",
+        "ratio_vs_plain": value / plain_rate,
+        "capacity_moves_per_s": capacity,
+        "capacity_ratio_vs_plain": capacity / plain_rate,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_batch<256>",
+            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_ms": st.kernel_ms_total / max(st.ticks, 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

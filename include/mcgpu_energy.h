@@ -182,6 +182,11 @@ typedef struct mcgpu_box_stats {
   double energy;  /* running total energy (start energy + accepted dE) */
   double start_energy;
   double box[3];  /* current edges (change under NPT) */
+  /* Work done so far: candidate molecules examined and sites loaded (in-range
+   * molecules' sites plus the moving molecule's old and proposed sites).
+   * Algorithmic bytes = 24 * (work_candidates + work_sites). */
+  int64_t work_candidates;
+  int64_t work_sites;
   int32_t done;
   int32_t fault;  /* nonzero: box stopped (see mcgpu_last_error on add) */
 } mcgpu_box_stats;

@@ -128,6 +128,8 @@ typedef struct mcgpu_server_stats {
   double kernel_ms_total;
   int64_t peak_resident;
   size_t peak_bytes;
+  int64_t work_candidates; /* summed over retired jobs (see mcgpu_box_stats) */
+  int64_t work_sites;
 } mcgpu_server_stats;
 int mcgpu_server_get_stats(mcgpu_server* s, mcgpu_server_stats* out);
 

@@ -65,7 +65,8 @@ class BoxStats(C.Structure):
         ("steps_done", C.c_int64), ("accepted", C.c_int64),
         ("volume_moves", C.c_int64), ("volume_accepted", C.c_int64),
         ("energy", C.c_double), ("start_energy", C.c_double),
-        ("box", C.c_double * 3), ("done", C.c_int32), ("fault", C.c_int32),
+        ("box", C.c_double * 3), ("work_candidates", C.c_int64),
+        ("work_sites", C.c_int64), ("done", C.c_int32), ("fault", C.c_int32),
     ]
 
 
@@ -106,7 +107,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("wall_s", C.c_double), ("total_moves", C.c_int64), ("ticks", C.c_int64),
         ("kernel_ms_total", C.c_double), ("peak_resident", C.c_int64),
-        ("peak_bytes", C.c_size_t),
+        ("peak_bytes", C.c_size_t), ("work_candidates", C.c_int64),
+        ("work_sites", C.c_int64),
     ]
 
 

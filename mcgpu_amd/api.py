@@ -346,6 +346,7 @@ class Server:
         check(lib().mcgpu_server_create(C.byref(cfg), C.byref(self._s)),
               "mcgpu_server_create")
         self.n = 0
+        self._boxes = []
 
     def close(self):
         if self._s:
@@ -369,6 +370,7 @@ class Server:
                       volume_interval, steps, pressure, max_volume_delta, e.value,
                       neighbor, 0)
         jid = C.c_int64()
+        self._boxes.append(box)   # the server reads it until run() returns
         check(lib().mcgpu_server_submit(self._s, box.handle, C.byref(p),
                                         C.c_double(arrival_s), C.byref(jid)),
               "mcgpu_server_submit")

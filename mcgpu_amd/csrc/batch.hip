@@ -34,6 +34,9 @@ struct BoxCtl {
   int64_t accepted;
   int64_t steps_target;
   int64_t next_stop;  // the kernel stops this box when steps_done reaches it
+  int64_t work_cand;   // candidate molecules examined (sum over moves)
+  int64_t work_sites;  // sites of in-range molecules loaded
+  int64_t work_self;   // sites of the moving molecule (old + proposed)
   double energy;
   int32_t status;  // kRunning / kDone / kVolumePending
   int32_t pad;
@@ -166,11 +169,15 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
     __syncthreads();
     propose_device<NT>(sh, bv, mv.vertex, mv.d, mv.cs, mv.sn);
     double eo, en;
+    EvalWork w;
     if (B.precision == MCGPU_MIXED)
-      eval_molecule<NT, true>(bv, mol, 0, true, sh, eo, en, nullptr);
+      eval_molecule<NT, true>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
     else
-      eval_molecule<NT, false>(bv, mol, 0, true, sh, eo, en, nullptr);
+      eval_molecule<NT, false>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
     if (t == 0) {
+      c.work_cand += w.cand;
+      c.work_sites += w.sites;
+      c.work_self += 2 * sh.len;
       const double de = en - eo;
       int acc;
       if (de < 0) {
@@ -839,6 +846,8 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   out->volume_accepted = b->slots[s].vol_acc;
   out->energy = c.energy;
   out->start_energy = b->slots[s].start_energy;
+  out->work_candidates = c.work_cand;
+  out->work_sites = c.work_sites + c.work_self;
   for (int k = 0; k < 3; k++) out->box[k] = b->slots[s].box.bv.L[k];
   out->done = c.status == kDone;
   out->fault = b->slots[s].fault;

@@ -27,6 +27,8 @@ struct EvalShared {
   int32_t cand_j[kCap];
   int32_t cand_f[kCap];  // bit0: in range at old position, bit1: at new
   double red[NT / 64];
+  double red2[NT / 64];
+  unsigned long long redc[NT / 64];
   int32_t scan[NT / 64];
   int32_t npri;
   int32_t len;
@@ -86,7 +88,8 @@ template <int NT, bool MIXED>
 __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
                                            int nhits, double& e_old,
                                            double& e_new,
-                                           unsigned long long& npairs) {
+                                           unsigned long long& npairs,
+                                           unsigned& nsites) {
   const int M = bv.max_len;
   const int len = sh.len;
   for (int item = threadIdx.x; item < nhits * M; item += NT) {
@@ -95,6 +98,7 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
     const int j = sh.cand_j[k];
     if (b >= bv.mol_len[j]) continue;
     const int f = sh.cand_f[k];
+    nsites++;
     const int site = bv.mol_start[j] + b;
     const int pi = bv.mol_param[j] + b;
     const double xb = bv.x[site], yb = bv.y[site], zb = bv.z[site];
@@ -162,10 +166,19 @@ __device__ __forceinline__ int pick_source(const BoxView& bv, const double* ox,
 // position, and at the proposed position when has_new.  Must be called by
 // all NT threads after load_molecule (+ proposed coords) and a barrier.
 // Results are block-uniform.
+// Work counters of one evaluation (block-uniform): candidate molecules
+// examined and sites of in-range molecules loaded; they give the kernel's
+// algorithmic bytes (24 B per candidate primary, 24 B per site).
+struct EvalWork {
+  unsigned cand;
+  unsigned sites;
+};
+
 template <int NT, bool MIXED>
 __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
                               bool has_new, EvalShared<NT>& sh, double& e_old,
-                              double& e_new, unsigned long long* pair_count) {
+                              double& e_new, unsigned long long* pair_count,
+                              EvalWork* work = nullptr) {
   const int t = threadIdx.x;
   const int p0 = sh.pofs[0];
   const int src = pick_source(bv, sh.ox, sh.oy, sh.oz, sh.nx, sh.ny, sh.nz, p0,
@@ -197,6 +210,7 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
 
   double eo = 0.0, en = 0.0;
   unsigned long long np = 0;
+  unsigned ncand = 0, nsites = 0;
   int nhits = 0;
   for (int base = 0; base < nitems; base += NT) {
     const int item = base + t;
@@ -215,6 +229,7 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
     }
     int f = 0;
     if (j >= jlo && j != mol) {
+      ncand++;
       f = primary_in_range(bv, j, sh.ox, sh.oy, sh.oz, sh.pofs, sh.npri) ? 1 : 0;
       if (has_new)
         f |= primary_in_range(bv, j, sh.nx, sh.ny, sh.nz, sh.pofs, sh.npri) ? 2 : 0;
@@ -222,7 +237,7 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
     int total;
     const int pos = block_scan_flag<NT>(f != 0, sh.scan, total);
     if (nhits + total > EvalShared<NT>::kCap) {
-      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np);
+      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np, nsites);
       nhits = 0;
     }
     if (f) {
@@ -232,9 +247,36 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
     nhits += total;
     __syncthreads();
   }
-  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np);
-  e_old = block_sum<NT>(eo, sh.red);
-  e_new = has_new ? block_sum<NT>(en, sh.red) : 0.0;
+  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np, nsites);
+  // one fixed-order reduction of both energies and the work counters
+  unsigned long long wc = ((unsigned long long)nsites << 32) | ncand;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    eo += __shfl_xor(eo, o);
+    en += __shfl_xor(en, o);
+    wc += __shfl_xor(wc, o);
+  }
+  if (lane_id() == 0) {
+    sh.red[wave_id()] = eo;
+    sh.red2[wave_id()] = en;
+    sh.redc[wave_id()] = wc;
+  }
+  __syncthreads();
+  double so = 0.0, sn = 0.0;
+  unsigned long long sc = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    so += sh.red[w];
+    sn += sh.red2[w];
+    sc += sh.redc[w];
+  }
+  __syncthreads();
+  e_old = so;
+  e_new = has_new ? sn : 0.0;
+  if (work) {
+    work->cand = (unsigned)(sc & 0xffffffffu);
+    work->sites = (unsigned)(sc >> 32);
+  }
   if (pair_count) {
     const double c = block_sum<NT>((double)np, sh.red);
     if (t == 0) *pair_count = (unsigned long long)c;

@@ -1,0 +1,185 @@
+// Serving loop: continuous batching of simulation jobs over one device batch.
+//
+// Jobs arrive over time (arrival_s on the server clock).  Between persistent
+// launches ("ticks") the loop admits every arrived job whose device
+// footprint fits the HBM budget and a free slot (FIFO), then launches one
+// tick that advances every resident box by up to sweeps_per_tick moves, then
+// retires finished boxes, freeing their slot and memory for the next
+// admissions.  Box state and RNG stay on the device from admission to
+// retirement.  Latency of a job = finish time - arrival time.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hostbox.h"
+#include "mcgpu_host.h"
+
+namespace mcgpu {
+HostBox& hostbox_ref(mcgpu_hostbox* h);
+}
+
+using mcgpu::HostBox;
+
+namespace {
+
+struct Job {
+  mcgpu_hostbox* box;  // owned by the caller until mcgpu_server_run returns
+  mcgpu_run_params params;
+  mcgpu_job_result res;
+  int32_t slot = -1;
+};
+
+}  // namespace
+
+struct mcgpu_server {
+  mcgpu_server_config cfg{};
+  mcgpu_batch* batch = nullptr;
+  std::vector<Job> jobs;
+  mcgpu_server_stats stats{};
+  bool ran = false;
+};
+
+extern "C" int mcgpu_server_create(const mcgpu_server_config* cfg,
+                                   mcgpu_server** out) {
+  if (!cfg || !out || cfg->max_boxes < 1 || cfg->sweeps_per_tick < 1) {
+    mcgpu::set_error("mcgpu_server_create: bad configuration");
+    return MCGPU_ERR_INVALID;
+  }
+  auto* s = new (std::nothrow) mcgpu_server();
+  if (!s) return MCGPU_ERR_NOMEM;
+  s->cfg = *cfg;
+  int rc = mcgpu_batch_create(cfg->max_boxes, cfg->hbm_budget_bytes, &s->batch);
+  if (rc != MCGPU_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_destroy(mcgpu_server* s) {
+  if (!s) return MCGPU_OK;
+  if (s->batch) mcgpu_batch_destroy(s->batch);
+  delete s;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_submit(mcgpu_server* s, mcgpu_hostbox* box,
+                                   const mcgpu_run_params* params,
+                                   double arrival_s, int64_t* job_id) {
+  if (!s || !box || !params || s->ran) {
+    mcgpu::set_error("mcgpu_server_submit: bad argument or server already ran");
+    return MCGPU_ERR_INVALID;
+  }
+  Job j;
+  j.box = box;
+  j.params = *params;
+  std::memset(&j.res, 0, sizeof(j.res));
+  j.res.job_id = (int64_t)s->jobs.size();
+  j.res.arrival_s = arrival_s;
+  j.res.steps = params->steps;
+  s->jobs.push_back(j);
+  if (job_id) *job_id = j.res.job_id;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_run(mcgpu_server* s) {
+  if (!s || s->ran) return MCGPU_ERR_INVALID;
+  s->ran = true;
+  std::vector<size_t> order(s->jobs.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return s->jobs[a].res.arrival_s < s->jobs[b].res.arrival_s;
+  });
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+  std::vector<int32_t> resident;  // job indices on the device
+  size_t next = 0, completed = 0;
+  int64_t peak = 0;
+  mcgpu_box_stats st;
+  while (completed < s->jobs.size()) {
+    // admit every arrived job that fits (FIFO; the head waits for memory)
+    const double t = now();
+    while (next < order.size() && s->jobs[order[next]].res.arrival_s <= t) {
+      Job& j = s->jobs[order[next]];
+      const mcgpu_box_desc d = mcgpu::hostbox_ref(j.box).desc();
+      int32_t slot = -1;
+      const int rc = mcgpu_batch_add(s->batch, &d, &j.params, &slot);
+      if (rc == MCGPU_ERR_NOMEM && !resident.empty()) break;  // wait for room
+      next++;
+      if (rc != MCGPU_OK) {  // failed job: isolated, reported, not retried
+        j.res.status = rc;
+        j.res.admit_s = j.res.finish_s = now();
+        completed++;
+        continue;
+      }
+      j.slot = slot;
+      j.res.admit_s = now();
+      resident.push_back((int32_t)order[next - 1]);
+    }
+    peak = std::max<int64_t>(peak, (int64_t)resident.size());
+    if (resident.empty()) {
+      if (next < order.size()) {
+        const double wait = s->jobs[order[next]].res.arrival_s - now();
+        if (wait > 0)
+          std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+      }
+      continue;
+    }
+    int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 1);
+    if (rc != MCGPU_OK) return rc;
+    s->stats.ticks++;
+    // retire finished jobs
+    for (size_t k = 0; k < resident.size();) {
+      Job& j = s->jobs[resident[k]];
+      rc = mcgpu_batch_stats(s->batch, j.slot, &st);
+      if (rc != MCGPU_OK) return rc;
+      if (st.done) {
+        j.res.finish_s = now();
+        j.res.final_energy = st.energy;
+        j.res.start_energy = st.start_energy;
+        j.res.accepted = st.accepted;
+        j.res.steps = st.steps_done;
+        j.res.status = st.fault ? MCGPU_ERR_STATE : MCGPU_OK;
+        s->stats.total_moves += st.steps_done;
+        s->stats.work_candidates += st.work_candidates;
+        s->stats.work_sites += st.work_sites;
+        rc = mcgpu_batch_remove(s->batch, j.slot);
+        if (rc != MCGPU_OK) return rc;
+        j.slot = -1;
+        completed++;
+        resident[k] = resident.back();
+        resident.pop_back();
+      } else {
+        k++;
+      }
+    }
+  }
+  s->stats.wall_s = now();
+  s->stats.peak_resident = peak;
+  double avg_ms = 0;
+  int64_t launches = 0;
+  if (mcgpu_batch_kernel_ms(s->batch, &avg_ms, &launches) == MCGPU_OK)
+    s->stats.kernel_ms_total = avg_ms * launches;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out,
+                                    int64_t cap, int64_t* n) {
+  if (!s || !n) return MCGPU_ERR_INVALID;
+  const int64_t k = std::min<int64_t>(cap, (int64_t)s->jobs.size());
+  for (int64_t i = 0; i < k; i++) out[i] = s->jobs[i].res;
+  *n = k;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_get_stats(mcgpu_server* s, mcgpu_server_stats* out) {
+  if (!s || !out) return MCGPU_ERR_INVALID;
+  *out = s->stats;
+  return MCGPU_OK;
+}

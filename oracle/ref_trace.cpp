@@ -13,7 +13,10 @@
 //   step loop                   Simulation.cpp:219-264
 // and, per move, prints one line: step mol oldE newE dE accepted.
 //
-// usage: ref_trace <input .state|.config> <steps> <trace-out> [final-coords-out]
+// usage: ref_trace <input .state|.config> <steps> <trace-out|-> [final-coords-out]
+// With "-" no trace is written; the step loop's wall time is printed
+// ("Loop Seconds") so the reference can serve as a timed CPU baseline.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -86,12 +89,15 @@ int main(int argc, char** argv) {
   Real e0_pairs = energy;
   energy += lrc;
 
-  FILE* out = std::fopen(argv[3], "w");
-  if (!out) return 1;
-  std::fprintf(out, "# seed %d molecules %d sites %d\n",
+  const bool tracing = std::string(argv[3]) != "-";
+  FILE* out = tracing ? std::fopen(argv[3], "w") : nullptr;
+  if (tracing && !out) return 1;
+  FILE* hdr = out ? out : stdout;
+  std::fprintf(hdr, "# seed %d molecules %d sites %d\n",
                box->environment->randomseed, sb->numMolecules, sb->numAtoms);
-  std::fprintf(out, "# E0_pairs %.17g LRC %.17g E0 %.17g\n", e0_pairs, lrc,
+  std::fprintf(hdr, "# E0_pairs %.17g LRC %.17g E0 %.17g\n", e0_pairs, lrc,
                energy);
+  const auto t0 = std::chrono::steady_clock::now();
   long accepted = 0, rejected = 0;
   for (long move = stepStart; move < stepStart + simSteps; move++) {
     ensemble.update();
@@ -108,12 +114,16 @@ int main(int argc, char** argv) {
       step->rollback(idx, sb);
     }
     sb->stepNum++;
-    std::fprintf(out, "%ld %d %.17g %.17g %.17g %d\n", move, idx, oldE, newE,
-                 newE - oldE, acc);
+    if (out)
+      std::fprintf(out, "%ld %d %.17g %.17g %.17g %d\n", move, idx, oldE, newE,
+                   newE - oldE, acc);
   }
-  std::fprintf(out, "# final %.17g accepted %ld rejected %ld\n", energy,
+  const double loop_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fprintf(hdr, "# final %.17g accepted %ld rejected %ld\n", energy,
                accepted, rejected);
-  std::fclose(out);
+  if (out) std::fclose(out);
+  std::printf("Loop Seconds: %.6f\n", loop_s);
 
   if (argc > 4) {
     FILE* fc = std::fopen(argv[4], "w");

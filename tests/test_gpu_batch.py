@@ -20,15 +20,25 @@ def load(name):
 
 
 def de_ok(got, want):
-    return np.all(np.abs(got - want) <= DE_TOL + 1e-14 * np.abs(want))
+    """1e-9 kcal/mol plus 1e-12 |dE|.  The batched path builds trial moves on
+    the device, whose sin/cos may differ from glibc's in the last bit; at
+    trial positions overlapping another molecule (|dE| up to 1e12 kcal/mol
+    in the generated 16-site boxes) that bit moves dE by ~1e-13 |dE|.  On the
+    256-water box every move meets the absolute 1e-9 bound."""
+    err = np.abs(got - want)
+    print(f"max |dE err| {err.max():.3e}, max rel {np.max(err / np.maximum(np.abs(want), 1e-300)):.3e}")
+    return np.all(err <= DE_TOL + 1e-12 * np.abs(want))
 
 
 def run_all(batch, slots, total, chunk):
-    done = 0
-    while done < total:
-        batch.run(min(chunk, total - done))
-        done += chunk
-    return [batch.stats(s) for s in slots]
+    """Runs until every box is done (volume moves pause a box between
+    launches, so more launches than total/chunk may be needed)."""
+    for _ in range(100 * (total // chunk + 1)):
+        stats = [batch.stats(s) for s in slots]
+        if all(st.done for st in stats):
+            return stats
+        batch.run(chunk)
+    raise AssertionError("boxes did not finish")
 
 
 def test_batch_fp64_reproduces_reference_trace():
