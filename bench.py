@@ -119,9 +119,10 @@ def run_kwargs(job, box):
 
 
 def serve(jobs, pool, t_shift=True, sweeps=200, max_boxes=1024,
-          budget=64 << 30):
+          budget=64 << 30, tick_us=0.0):
     import mcgpu_amd as m
-    srv = m.Server(max_boxes=max_boxes, sweeps_per_tick=sweeps, hbm_budget=budget)
+    srv = m.Server(max_boxes=max_boxes, sweeps_per_tick=sweeps, hbm_budget=budget,
+                   tick_budget_us=tick_us)
     t0 = jobs[0]["arrival"] if t_shift else 0.0
     for j in jobs:
         box = pool.get(j["kind"], j["n"])
@@ -205,7 +206,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="jobs in the timed stream")
     ap.add_argument("--warmup", type=int, default=32, help="warm-up jobs (untimed)")
-    ap.add_argument("--sweeps", type=int, default=200, help="moves per box per tick")
+    ap.add_argument("--sweeps", type=int, default=2000, help="max moves per box per tick")
+    ap.add_argument("--tick-us", type=float, default=2000.0,
+                    help="per-launch time limit in microseconds")
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -230,13 +233,15 @@ def main() -> int:
     print(f"# generated {len(pool.boxes)} boxes for {len(stream)} jobs in "
           f"{setup_s:.1f} s", file=sys.stderr, flush=True)
 
+    os.environ.setdefault("MCGPU_SERVER_LOG", "10")
     # warm-up: same server path, saturated (all warm-up jobs at t=0)
     wj = [dict(j, arrival=0.0, steps=min(j["steps"], 2000)) for j in warm]
-    serve(wj, pool, t_shift=False, sweeps=args.sweeps)
+    serve(wj, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
+    print("# warm-up done", file=sys.stderr, flush=True)
 
     # timed region: exactly K jobs of the stream
     t0 = time.perf_counter()
-    res, st = serve(timed, pool, sweeps=args.sweeps)
+    res, st = serve(timed, pool, sweeps=args.sweeps, tick_us=args.tick_us)
     wall = time.perf_counter() - t0
     bad = [r for r in res if r.status != 0]
     if bad:
@@ -252,8 +257,9 @@ def main() -> int:
 
     # saturated capacity: the same jobs, all arriving at t = 0
     cap_jobs = [dict(j, arrival=0.0) for j in timed[: min(len(timed), 600)]]
-    _, cst = serve(cap_jobs, pool, t_shift=False, sweeps=args.sweeps)
+    _, cst = serve(cap_jobs, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
     capacity = cst.total_moves / cst.wall_s
+    print(f"# capacity run: {capacity:.4g} moves/s", file=sys.stderr, flush=True)
 
     # roofline of the batch kernel (timed serving run)
     kernel_s = st.kernel_ms_total / 1e3
@@ -288,7 +294,8 @@ def main() -> int:
             "workload": ("serving stream: Poisson 200 jobs/s, water (c2 generator) "
                          "and 16-site bicyclic (c1 generator) boxes, 128-16384 "
                          "molecules, 1e3-1e5 steps, NVT/NPT, FP64/FP32-pair mixed"),
-            "jobs": len(timed), "sweeps_per_tick": args.sweeps,
+            "jobs": len(timed), "max_sweeps_per_tick": args.sweeps,
+            "tick_budget_us": args.tick_us,
             "parallelism": "boxes in one persistent launch",
         },
         "p50_latency_s": float(np.median(lat)),

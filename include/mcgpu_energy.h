@@ -155,6 +155,11 @@ int mcgpu_box_size(mcgpu_box* box, double* edges3);
 int mcgpu_box_enable_timing(mcgpu_box* box, int on);
 int mcgpu_last_kernel_ms(mcgpu_box* box, double* ms);
 
+/* Test hook (no counterpart in the reference): evaluates the device sin,
+ * cos and exp used for trial-move rotations and the Metropolis test. */
+int mcgpu_debug_math(const double* x, int32_t n, double* sin_out,
+                     double* cos_out, double* exp_out);
+
 /* ------------------------------------------------------------------ */
 /* Batched boxes: many independent boxes advanced by one persistent launch.
  * Each box carries its own device RNG (a glibc TYPE_3 stream seeded like
@@ -216,6 +221,16 @@ int mcgpu_batch_download(mcgpu_batch* b, int32_t slot, double* x, double* y,
 int mcgpu_batch_trace(mcgpu_batch* b, int32_t slot, int32_t* mol,
                       double* delta, int8_t* accepted, int64_t cap,
                       int64_t* n);
+/* Per-launch time limit (microseconds, 0 = none): a box stops its sweeps
+ * when its work-group has run this long, so small boxes do not wait idle
+ * for the slowest one.  Results do not depend on it (each box's move
+ * sequence is fixed by its RNG stream), only how moves split over launches. */
+int mcgpu_batch_set_tick_budget(mcgpu_batch* b, double microseconds);
+/* Arena bytes in use, peak, and capacity (the admission budget). */
+int mcgpu_batch_memory(mcgpu_batch* b, size_t* used, size_t* peak,
+                       size_t* capacity);
+/* Work-group size the batch chose for a box (256 or 512 lanes). */
+int mcgpu_batch_group(mcgpu_batch* b, int32_t slot, int32_t* threads);
 /* Average device duration (ms) of the batch kernel over the last
  * mcgpu_batch_run calls since the previous query, and their count. */
 int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms, int64_t* launches);

@@ -91,8 +91,9 @@ int mcgpu_strategy_run(mcgpu_hostbox* h, int64_t steps, int32_t* trace_mol,
 /* ---- serving loop ---- */
 typedef struct mcgpu_server_config {
   int32_t max_boxes;      /* resident slots */
-  int32_t sweeps_per_tick;/* moves per box per persistent launch */
+  int32_t sweeps_per_tick;/* at most this many moves per box per launch */
   size_t hbm_budget_bytes;/* admission budget */
+  double tick_budget_us;  /* per-launch time limit (0 = none) */
 } mcgpu_server_config;
 
 typedef struct mcgpu_job_result {
@@ -111,8 +112,9 @@ typedef struct mcgpu_job_result {
 typedef struct mcgpu_server mcgpu_server;
 int mcgpu_server_create(const mcgpu_server_config* cfg, mcgpu_server** out);
 int mcgpu_server_destroy(mcgpu_server* s);
-/* Queues a job (the server copies the box).  arrival_s: when it becomes
- * visible, on the server clock (seconds since mcgpu_server_start). */
+/* Queues a job.  The server reads `box` until mcgpu_server_run returns
+ * (the caller keeps it alive).  arrival_s: when the job becomes visible, on
+ * the server clock (seconds since mcgpu_server_run started). */
 int mcgpu_server_submit(mcgpu_server* s, mcgpu_hostbox* box,
                         const mcgpu_run_params* params, double arrival_s,
                         int64_t* job_id);

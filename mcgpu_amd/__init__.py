@@ -91,7 +91,7 @@ class PlainResult(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_boxes", C.c_int32), ("sweeps_per_tick", C.c_int32),
-                ("hbm_budget_bytes", C.c_size_t)]
+                ("hbm_budget_bytes", C.c_size_t), ("tick_budget_us", C.c_double)]
 
 
 class JobResult(C.Structure):

@@ -330,6 +330,14 @@ class Batch:
         k = n.value
         return RunTrace(tm[:k], td[:k], ta[:k])
 
+    def set_tick_budget(self, microseconds: float) -> None:
+        check(lib().mcgpu_batch_set_tick_budget(self._b, C.c_double(microseconds)))
+
+    def group_threads(self, slot: int) -> int:
+        t = C.c_int32()
+        check(lib().mcgpu_batch_group(self._b, int(slot), C.byref(t)))
+        return t.value
+
     def kernel_ms(self):
         ms = C.c_double()
         n = C.c_int64()
@@ -340,8 +348,9 @@ class Batch:
 class Server:
     """Continuous-batching request loop over device-resident boxes."""
 
-    def __init__(self, max_boxes: int, sweeps_per_tick: int, hbm_budget: int):
-        cfg = ServerConfig(max_boxes, sweeps_per_tick, hbm_budget)
+    def __init__(self, max_boxes: int, sweeps_per_tick: int, hbm_budget: int,
+                 tick_budget_us: float = 0.0):
+        cfg = ServerConfig(max_boxes, sweeps_per_tick, hbm_budget, tick_budget_us)
         self._s = C.c_void_p()
         check(lib().mcgpu_server_create(C.byref(cfg), C.byref(self._s)),
               "mcgpu_server_create")

@@ -23,6 +23,7 @@
 
 #include "box_internal.h"
 #include "eval.h"
+#include "fastmath.h"
 #include "mcgpu_host.h"
 
 namespace mcgpu {
@@ -38,8 +39,9 @@ struct BoxCtl {
   int64_t work_sites;  // sites of in-range molecules loaded
   int64_t work_self;   // sites of the moving molecule (old + proposed)
   double energy;
+  double start_energy;
   int32_t status;  // kRunning / kDone / kVolumePending
-  int32_t pad;
+  int32_t fault;   // MCGPU_FAULT_* that stopped the box (0 = none)
 };
 constexpr int32_t kRunning = 0, kDone = 1, kVolumePending = 2;
 
@@ -117,6 +119,7 @@ __device__ void propose_device(EvalShared<NT>& sh, const BoxView& bv, int vertex
 }
 
 struct MoveDraw {
+  int32_t go;
   int32_t mol;
   int32_t vertex;
   double d[3];
@@ -124,46 +127,55 @@ struct MoveDraw {
   double sn[3];
 };
 
+// The persistent launch: work-group g owns box active[g] and runs up to
+// `sweeps` trial moves on it, stopping early at the box's step target, at a
+// pending NPT volume move, or when `budget` wall-clock ticks have passed
+// since the work-group started (0 = no time limit).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
                                               BoxCtl* ctl,
                                               const int32_t* active,
-                                              int sweeps) {
+                                              int sweeps, long long budget) {
   __shared__ EvalShared<NT> sh;
-  __shared__ BatchBox B;
   __shared__ Rng31 rng;
   __shared__ MoveDraw mv;
   __shared__ BoxCtl c;
   const int t = threadIdx.x;
-  const int slot = active[blockIdx.x];
+  const int slot = __builtin_amdgcn_readfirstlane(active[blockIdx.x]);
+  const BatchBox& B = table[slot];
+  const long long t_start = wall_clock64();
   if (t == 0) {
-    B = table[slot];
     rng = *B.rng;
     c = ctl[slot];
   }
   __syncthreads();
   const BoxView& bv = B.bv;
   for (int s = 0; s < sweeps; s++) {
-    if (c.steps_done >= c.next_stop) break;  // block-uniform (LDS)
     if (t == 0) {
 #pragma clang fp contract(off)
-      const int n = bv.n_mol;
-      int mol = (int)random_real(rng31_next(rng), 0, n);
-      mol = mol >= n ? n - 1 : mol;  // rand() == RAND_MAX
-      const int len = bv.mol_len[mol];
-      int vtx = (int)random_real(rng31_next(rng), 0, len);
-      vtx = vtx >= len ? len - 1 : vtx;
-      mv.mol = mol;
-      mv.vertex = vtx;
-      for (int k = 0; k < 3; k++) mv.d[k] = random_real(rng31_next(rng), -B.max_t, B.max_t);
-      for (int k = 0; k < 3; k++) {
-        const double deg = random_real(rng31_next(rng), -B.max_r, B.max_r);
-        const double rad = deg * 3.14159265358979 / 180.0;
-        mv.cs[k] = cos(rad);
-        mv.sn[k] = sin(rad);
+      bool go = c.steps_done < c.next_stop;
+      if (go && budget > 0 && s > 0) go = (wall_clock64() - t_start) < budget;
+      mv.go = go;
+      if (go) {
+        const int n = bv.n_mol;
+        int mol = (int)random_real(rng31_next(rng), 0, n);
+        mol = mol >= n ? n - 1 : mol;  // rand() == RAND_MAX
+        const int len = bv.mol_len[mol];
+        int vtx = (int)random_real(rng31_next(rng), 0, len);
+        vtx = vtx >= len ? len - 1 : vtx;
+        mv.mol = mol;
+        mv.vertex = vtx;
+        for (int k = 0; k < 3; k++)
+          mv.d[k] = random_real(rng31_next(rng), -B.max_t, B.max_t);
+        for (int k = 0; k < 3; k++) {
+          const double deg = random_real(rng31_next(rng), -B.max_r, B.max_r);
+          const double rad = deg * 3.14159265358979 / 180.0;
+          sincos_fp64(rad, mv.sn[k], mv.cs[k]);
+        }
       }
     }
     __syncthreads();
+    if (!mv.go) break;
     const int mol = mv.mol;
     load_molecule<NT>(bv, mol, sh);
     __syncthreads();
@@ -174,6 +186,10 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
       eval_molecule<NT, true>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
     else
       eval_molecule<NT, false>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
+    if (sh.fault) {  // block-uniform (LDS): stop this box, report it
+      if (t == 0) c.fault = sh.fault;
+      break;
+    }
     if (t == 0) {
       c.work_cand += w.cand;
       c.work_sites += w.sites;
@@ -183,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
       if (de < 0) {
         acc = 1;
       } else {
-        acc = exp(-(de) / bv.kT) >= random_real(rng31_next(rng), 0.0, 1.0);
+        acc = exp_fp64(-(de) / bv.kT) >= random_real(rng31_next(rng), 0.0, 1.0);
       }
       if (B.record) {
         B.tr_mol[c.steps_done] = mol;
@@ -203,7 +219,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   }
   if (t == 0) {
     *B.rng = rng;
-    if (c.steps_done >= c.steps_target)
+    if (c.steps_done >= c.steps_target || c.fault)
       c.status = kDone;
     else if (c.steps_done >= c.next_stop)
       c.status = kVolumePending;
@@ -211,11 +227,24 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   }
 }
 
-__global__ void k_rng_draw(Rng31* rng, int n, int32_t* out) {
+// Start energy of a newly added box: pairs (reduced by k_reduce2 into
+// total[0]) plus the offset (the LRC).
+__global__ void k_set_start_energy(BoxCtl* c, const double* total, double offset) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    Rng31 g = *rng;
-    for (int i = 0; i < n; i++) out[i] = rng31_next(g);
-    *rng = g;
+    c->energy = total[0] + offset;
+    c->start_energy = c->energy;
+  }
+}
+
+// Copies the backup coordinates back after a rejected volume move.
+__global__ void k_restore_coords(double* x, double* y, double* z,
+                                 const double* bx, const double* by,
+                                 const double* bz, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = bx[i];
+    y[i] = by[i];
+    z[i] = bz[i];
   }
 }
 
@@ -285,12 +314,10 @@ class Arena {
 // Offsets of one box's arrays inside its arena block.
 struct BoxLayout {
   size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, pidx, p64, p32;
-  size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, rng, partial,
-      pairs, total, bx, by, bz, trm, trd, tra;
+  size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, partial, pairs,
+      total, bx, by, bz, trm, trd, tra;
   size_t bytes;
   int ncell, cap;
-  int cn[3];
-  double inv_w[3];
 };
 
 BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
@@ -319,15 +346,15 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   L.p32 = take(sizeof(SiteParam32) * pt.p32.size());
   L.ncell = 0;
   if (p.neighbor == MCGPU_NEIGHBOR_CELL) {
-    // room for the grid of a box up to 25% larger in volume (NPT)
+    // room for the grid of a box up to 26% larger in volume (NPT)
     double Lg[3];
     for (int k = 0; k < 3; k++) Lg[k] = d.box[k] * 1.08;
-    int n[3];
+    int n[3], cn[3];
     double iw[3];
     cell_geometry(Lg, d.cutoff, pt.spread, n, iw);
     L.ncell = n[0] * n[1] * n[2];
-    cell_geometry(d.box, d.cutoff, pt.spread, L.cn, L.inv_w);
-    const int nc_now = L.cn[0] * L.cn[1] * L.cn[2];
+    cell_geometry(d.box, d.cutoff, pt.spread, cn, iw);
+    const int nc_now = cn[0] * cn[1] * cn[2];
     L.cap = ((int)std::ceil(3.0 * nm / nc_now) + 16 + 3) & ~3;
     L.ccount = take(4 * (size_t)L.ncell);
     L.cslots = take(4 * (size_t)L.ncell * L.cap);
@@ -337,10 +364,9 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   L.covf = take(4);
   L.vstale = take(4);
   L.vcount = take(4);
-  L.rng = take(sizeof(Rng31));
   L.partial = take(8 * nm);
   L.pairs = take(8 * nm);
-  L.total = take(16);
+  L.total = take(32);  // [E_pairs, pairs] twice (before / after a rescale)
   if (p.ensemble == MCGPU_NPT) {
     L.bx = take(8 * ns);
     L.by = take(8 * ns);
@@ -355,6 +381,24 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   return L;
 }
 
+// Work-group size for a box: 512 lanes when one move has enough candidate
+// and pair work to keep them busy, else 256 (the estimate uses the
+// molecule density inside the cutoff sphere).
+int choose_group(const mcgpu_box_desc& d, int neighbor, int max_len) {
+  const double V = d.box[0] * d.box[1] * d.box[2];
+  const double rho = d.n_molecules / V;
+  const double rc = d.cutoff;
+  const double hits = std::min<double>(d.n_molecules - 1, 4.18879 * rc * rc * rc * rho);
+  double cand = d.n_molecules;
+  if (neighbor == MCGPU_NEIGHBOR_CELL) cand = std::min<double>(cand, 27.0 * rc * rc * rc * rho * 1.3);
+  int aw = 1;
+  while (aw < max_len) aw *= 2;
+  const double lane_steps = hits * max_len * aw;
+  return (cand + lane_steps / 2 > 3000.0) ? 1 : 0;
+}
+
+constexpr int kGroupNT[2] = {256, 512};
+
 }  // namespace mcgpu
 
 using namespace mcgpu;
@@ -363,38 +407,42 @@ struct BatchSlot {
   bool used = false;
   size_t off = 0;
   BoxLayout lay{};
-  BatchBox box{};      // host copy of the device table entry
   mcgpu_run_params params{};
   int32_t n_pidx = 0;
+  int32_t group = 0;
   double lrc = 0.0;    // current tail correction (scaled with 1/V under NPT)
   double spread = 0.0; // largest primary-to-first-primary distance
-  double start_energy = 0.0;
   int64_t vol_moves = 0, vol_acc = 0;
   int fault = 0;
+  // volume move in flight
+  double v_old = 0.0, v_new = 0.0, u_accept = 0.0, L_old[3] = {0, 0, 0};
 };
 
 struct mcgpu_batch {
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream = nullptr;  // main stream (group 0 and all setup)
+  hipStream_t stream2 = nullptr; // group 1 during a tick
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr, join = nullptr;
   Arena arena;
   int32_t max_boxes = 0;
   std::vector<BatchSlot> slots;
   BatchBox* d_table = nullptr;
+  BatchBox* h_table = nullptr;  // pinned mirror (the host's copy)
   BoxCtl* d_ctl = nullptr;
-  BoxCtl* h_ctl = nullptr;  // pinned mirror
-  int32_t* d_active = nullptr;
+  BoxCtl* h_ctl = nullptr;      // pinned mirror
+  Rng31* d_rng = nullptr;
+  Rng31* h_rng = nullptr;       // pinned mirror
+  int32_t* d_active = nullptr;  // [2 * max_boxes]: group 0, then group 1
   int32_t* h_active = nullptr;  // pinned
-  int32_t* d_draw = nullptr;
-  int32_t* h_draw = nullptr;  // pinned
-  double* h_total = nullptr;  // pinned [2]
-  int n_active = 0;
-  bool pending = false;  // a launch whose results were not yet consumed
-  bool ctl_dirty = false;
+  double* h_vol = nullptr;      // pinned [4 * max_boxes] volume-move energies
+  long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
+  double clock_khz = 100000.0;
+  bool pending = false;         // a launch whose results were not consumed
+  bool stale_ctl = false;       // h_ctl older than device-side changes
+  int n_active[2] = {0, 0};
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_sweeps = -1;
-  int graph_active = -1;
   uint64_t graph_version = 0, version = 1;
   // timing
   double kernel_ms_sum = 0.0;
@@ -404,10 +452,8 @@ struct mcgpu_batch {
 
 namespace {
 
-int batch_sync(mcgpu_batch* b);
-
 int upload_slot(mcgpu_batch* b, int s) {
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table + s, &b->slots[s].box, sizeof(BatchBox),
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table + s, b->h_table + s, sizeof(BatchBox),
                                hipMemcpyHostToDevice, b->stream));
   return MCGPU_OK;
 }
@@ -418,33 +464,39 @@ int upload_ctl(mcgpu_batch* b, int s) {
   return MCGPU_OK;
 }
 
-// Full-system energy of slot s (pairs only), synchronous.
-int slot_energy(mcgpu_batch* b, int s, double* e) {
-  BatchSlot& S = b->slots[s];
-  char* base = b->arena.base() + S.off;
-  int rc = launch_system_energy(S.box.bv, S.box.precision, (double*)(base + S.lay.partial),
-                                (double*)(base + S.lay.pairs), (double*)(base + S.lay.total),
-                                b->stream);
-  if (rc) return rc;
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_total, base + S.lay.total, 16,
+int readback(mcgpu_batch* b) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
                                hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  *e = b->h_total[0];
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng, b->d_rng, sizeof(Rng31) * b->max_boxes,
+                               hipMemcpyDeviceToHost, b->stream));
   return MCGPU_OK;
 }
 
-// Rebuilds slot s's cell list for its current box size; falls back to the
-// all-molecule scan when the grid no longer fits the allocation.
-int slot_rebuild_cells(mcgpu_batch* b, int s, double spread) {
+char* slot_base(mcgpu_batch* b, int s) { return b->arena.base() + b->slots[s].off; }
+
+// Full-system energy of slot s into total[which] (stream-ordered).
+int slot_energy_async(mcgpu_batch* b, int s, int which) {
   BatchSlot& S = b->slots[s];
-  BoxView& v = S.box.bv;
+  char* base = slot_base(b, s);
+  return launch_system_energy(b->h_table[s].bv, b->h_table[s].precision,
+                              (double*)(base + S.lay.partial),
+                              (double*)(base + S.lay.pairs),
+                              (double*)(base + S.lay.total) + 2 * which, b->stream);
+}
+
+// Cell grid for slot s's current box; the all-molecule scan when the grid
+// no longer fits the allocation.  Capacity overflow is detected on the
+// device (the kernels then scan every molecule).
+int slot_rebuild_cells(mcgpu_batch* b, int s) {
+  BatchSlot& S = b->slots[s];
+  BoxView& v = b->h_table[s].bv;
   if (S.lay.ncell == 0) {
     v.mode = MCGPU_NB_BRUTE;
     return MCGPU_OK;
   }
   int n[3];
   double iw[3];
-  cell_geometry(v.L, std::sqrt(v.cutoff2), spread, n, iw);
+  cell_geometry(v.L, std::sqrt(v.cutoff2), S.spread, n, iw);
   if (n[0] * n[1] * n[2] > S.lay.ncell) {
     v.mode = MCGPU_NB_BRUTE;
     return MCGPU_OK;
@@ -454,81 +506,94 @@ int slot_rebuild_cells(mcgpu_batch* b, int s, double spread) {
     v.cells.inv_w[k] = iw[k];
   }
   v.mode = MCGPU_NB_CELL;
-  int rc = launch_cells_build(v, b->stream);
-  if (rc) return rc;
-  int32_t ovf = 0;
-  MCGPU_HIP_TRY(hipMemcpyAsync(&ovf, v.cells.overflow, 4, hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  if (ovf) v.mode = MCGPU_NB_BRUTE;  // capacity exceeded: exact, slower scan
-  return MCGPU_OK;
+  return launch_cells_build(v, b->stream);
 }
 
-// NPT volume move for slot s (build semantics; the reference's NPT path,
-// Ensemble.cpp:9-28 / SimulationStep.cpp:476-518, always grows the box by
-// 1.1 and never recomputes the energy):
-//   V' = V (1 + delta (2 u1 - 1)), molecules translated so first primaries
+// NPT volume moves of every pending box, batched: all rescales and energies
+// are queued, one synchronisation collects them, the decisions are made and
+// rejected moves undone, and the updated descriptors are queued for upload.
+// Semantics (the build's own; the reference's NPT path, Ensemble.cpp:9-28 /
+// SimulationStep.cpp:476-518, always grows the box by 1.1 and never
+// recomputes the energy):
+//   V' = V (1 + delta (2 u1 - 1)); molecules translated so first primaries
 //   scale by (V'/V)^(1/3); dE = dE_pairs + LRC(V') - LRC(V);
 //   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2.
-int batch_volume_move(mcgpu_batch* b, int s) {
-  BatchSlot& S = b->slots[s];
-  BoxView& v = S.box.bv;
-  char* base = b->arena.base() + S.off;
-  BoxCtl& c = b->h_ctl[s];
+// u1 and u2 are the box's next two draws of its own RNG stream.
+int batch_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
-  double e_old;
-  if ((rc = slot_energy(b, s, &e_old))) return rc;
-  hipLaunchKernelGGL(k_rng_draw, dim3(1), dim3(64), 0, b->stream, S.box.rng, 2,
-                     b->d_draw);
-  MCGPU_HIP_TRY(hipGetLastError());
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_draw, b->d_draw, 8, hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  const double u1 = random_real(b->h_draw[0], 0.0, 1.0);
-  const double u2 = random_real(b->h_draw[1], 0.0, 1.0);
-  const double v_old = v.L[0] * v.L[1] * v.L[2];
-  const double v_new = v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
-  const double f = std::cbrt(v_new / v_old);
-  const size_t nb = 8 * (size_t)v.n_site;
-  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bx, v.x, nb, hipMemcpyDeviceToDevice, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.by, v.y, nb, hipMemcpyDeviceToDevice, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bz, v.z, nb, hipMemcpyDeviceToDevice, b->stream));
-  double L_old[3];
-  for (int k = 0; k < 3; k++) L_old[k] = v.L[k];
-  if ((rc = launch_volume_scale(v, f, b->stream))) return rc;
-  for (int k = 0; k < 3; k++) v.L[k] = L_old[k] * f;
-  if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
-  if ((rc = slot_rebuild_cells(b, s, S.spread))) return rc;
-  double e_new;
-  if ((rc = slot_energy(b, s, &e_new))) return rc;
-  const double lrc_new = S.lrc * v_old / v_new;
-  const double de = (e_new - e_old) + (lrc_new - S.lrc);
-  double prob = v.n_mol * (std::log(v_new) - std::log(v_old));
-  prob -= (de + S.params.pressure * (v_new - v_old)) / v.kT;
-  prob = std::exp(prob);
-  S.vol_moves++;
-  if (prob >= u2) {
-    S.vol_acc++;
-    c.energy += de;
-    S.lrc = lrc_new;
-  } else {
-    MCGPU_HIP_TRY(hipMemcpyAsync(v.x, base + S.lay.bx, nb, hipMemcpyDeviceToDevice, b->stream));
-    MCGPU_HIP_TRY(hipMemcpyAsync(v.y, base + S.lay.by, nb, hipMemcpyDeviceToDevice, b->stream));
-    MCGPU_HIP_TRY(hipMemcpyAsync(v.z, base + S.lay.bz, nb, hipMemcpyDeviceToDevice, b->stream));
-    for (int k = 0; k < 3; k++) v.L[k] = L_old[k];
+  for (int s : pend) {
+    BatchSlot& S = b->slots[s];
+    BoxView& v = b->h_table[s].bv;
+    char* base = slot_base(b, s);
+    Rng31& g = b->h_rng[s];
+    const double u1 = random_real(rng31_next(g), 0.0, 1.0);
+    S.u_accept = random_real(rng31_next(g), 0.0, 1.0);
+    for (int k = 0; k < 3; k++) S.L_old[k] = v.L[k];
+    S.v_old = v.L[0] * v.L[1] * v.L[2];
+    S.v_new = S.v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
+    const double f = std::cbrt(S.v_new / S.v_old);
+    if ((rc = slot_energy_async(b, s, 0))) return rc;
+    const size_t nb = 8 * (size_t)v.n_site;
+    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bx, v.x, nb, hipMemcpyDeviceToDevice, b->stream));
+    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.by, v.y, nb, hipMemcpyDeviceToDevice, b->stream));
+    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bz, v.z, nb, hipMemcpyDeviceToDevice, b->stream));
+    if ((rc = launch_volume_scale(v, f, b->stream))) return rc;
+    for (int k = 0; k < 3; k++) v.L[k] = S.L_old[k] * f;
     if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
-    if ((rc = slot_rebuild_cells(b, s, S.spread))) return rc;
+    if ((rc = slot_rebuild_cells(b, s))) return rc;
+    if ((rc = slot_energy_async(b, s, 1))) return rc;
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->h_vol + 4 * s, base + S.lay.total, 32,
+                                 hipMemcpyDeviceToHost, b->stream));
   }
-  c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
-  c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
-  if ((rc = upload_slot(b, s))) return rc;
-  if ((rc = upload_ctl(b, s))) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int s : pend) {
+    BatchSlot& S = b->slots[s];
+    BoxView& v = b->h_table[s].bv;
+    BoxCtl& c = b->h_ctl[s];
+    const double e_old = b->h_vol[4 * s], e_new = b->h_vol[4 * s + 2];
+    const double lrc_new = S.lrc * S.v_old / S.v_new;
+    const double de = (e_new - e_old) + (lrc_new - S.lrc);
+    double prob = v.n_mol * (std::log(S.v_new) - std::log(S.v_old));
+    prob -= (de + S.params.pressure * (S.v_new - S.v_old)) / v.kT;
+    prob = std::exp(prob);
+    S.vol_moves++;
+    if (prob >= S.u_accept) {
+      S.vol_acc++;
+      c.energy += de;
+      S.lrc = lrc_new;
+    } else {
+      char* base = slot_base(b, s);
+      hipLaunchKernelGGL(k_restore_coords, dim3((v.n_site + 255) / 256), dim3(256), 0,
+                         b->stream, v.x, v.y, v.z, (const double*)(base + S.lay.bx),
+                         (const double*)(base + S.lay.by), (const double*)(base + S.lay.bz),
+                         v.n_site);
+      MCGPU_HIP_TRY(hipGetLastError());
+      for (int k = 0; k < 3; k++) v.L[k] = S.L_old[k];
+      if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
+      if ((rc = slot_rebuild_cells(b, s))) return rc;
+    }
+    c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
+    c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
+    if ((rc = upload_slot(b, s))) return rc;
+    if ((rc = upload_ctl(b, s))) return rc;
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_rng + s, b->h_rng + s, sizeof(Rng31),
+                                 hipMemcpyHostToDevice, b->stream));
+  }
   b->version++;
   return MCGPU_OK;
 }
 
-// Consumes a finished launch: reads timing, runs pending volume moves.
+// Consumes a finished launch: reads timing and control state, runs pending
+// volume moves.
 int batch_sync(mcgpu_batch* b) {
-  if (!b->pending) return MCGPU_OK;
+  if (!b->pending && !b->stale_ctl) return MCGPU_OK;
+  if (b->stale_ctl && !b->pending) {
+    int rc = readback(b);
+    if (rc) return rc;
+  }
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->stale_ctl = false;
+  const bool launched = b->pending;
   b->pending = false;
   if (b->timing_pending) {
     float ms = 0.f;
@@ -537,13 +602,11 @@ int batch_sync(mcgpu_batch* b) {
     b->kernel_launches++;
     b->timing_pending = false;
   }
-  for (int s = 0; s < b->max_boxes; s++) {
-    if (!b->slots[s].used) continue;
-    if (b->h_ctl[s].status == kVolumePending) {
-      int rc = batch_volume_move(b, s);
-      if (rc) return rc;
-    }
-  }
+  if (!launched) return MCGPU_OK;
+  std::vector<int> pend;
+  for (int s = 0; s < b->max_boxes; s++)
+    if (b->slots[s].used && b->h_ctl[s].status == kVolumePending) pend.push_back(s);
+  if (!pend.empty()) return batch_volume_moves(b, pend);
   return MCGPU_OK;
 }
 
@@ -579,23 +642,34 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
     return code;
   };
   if ((rc = b->arena.init(budget))) return fail(rc);
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+      khz > 0)
+    b->clock_khz = khz;
+  const size_t M = max_boxes;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipMalloc(&b->d_table, sizeof(BatchBox) * max_boxes) != hipSuccess ||
-      hipMalloc(&b->d_ctl, sizeof(BoxCtl) * max_boxes) != hipSuccess ||
-      hipMalloc(&b->d_active, sizeof(int32_t) * max_boxes) != hipSuccess ||
-      hipMalloc(&b->d_draw, 64) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * max_boxes, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * max_boxes, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_draw, 64, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_total, 16, 0) != hipSuccess) {
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&b->d_table, sizeof(BatchBox) * M) != hipSuccess ||
+      hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) != hipSuccess ||
+      hipMalloc(&b->d_rng, sizeof(Rng31) * M) != hipSuccess ||
+      hipMalloc(&b->d_active, sizeof(int32_t) * 2 * M) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * 2 * M, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_vol, sizeof(double) * 4 * M, 0) != hipSuccess) {
     set_error("mcgpu_batch_create: allocation failed");
     return fail(MCGPU_ERR_NOMEM);
   }
-  std::memset(b->h_ctl, 0, sizeof(BoxCtl) * max_boxes);
-  for (int s = 0; s < max_boxes; s++) b->h_ctl[s].status = kDone;
-  if (hipMemcpy(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * max_boxes, hipMemcpyHostToDevice) !=
-      hipSuccess) {
+  std::memset(b->h_table, 0, sizeof(BatchBox) * M);
+  std::memset(b->h_ctl, 0, sizeof(BoxCtl) * M);
+  std::memset(b->h_rng, 0, sizeof(Rng31) * M);
+  for (size_t s = 0; s < M; s++) b->h_ctl[s].status = kDone;
+  if (hipMemcpy(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * M, hipMemcpyHostToDevice) != hipSuccess) {
     set_error("mcgpu_batch_create: upload failed");
     return fail(MCGPU_ERR_HIP);
   }
@@ -606,20 +680,32 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
 extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (!b) return MCGPU_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_table) (void)hipFree(b->d_table);
   if (b->d_ctl) (void)hipFree(b->d_ctl);
+  if (b->d_rng) (void)hipFree(b->d_rng);
   if (b->d_active) (void)hipFree(b->d_active);
-  if (b->d_draw) (void)hipFree(b->d_draw);
+  if (b->h_table) (void)hipHostFree(b->h_table);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  if (b->h_rng) (void)hipHostFree(b->h_rng);
   if (b->h_active) (void)hipHostFree(b->h_active);
-  if (b->h_draw) (void)hipHostFree(b->h_draw);
-  if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->h_vol) (void)hipHostFree(b->h_vol);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   delete b;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_set_tick_budget(mcgpu_batch* b, double microseconds) {
+  if (!b || microseconds < 0) return MCGPU_ERR_INVALID;
+  b->budget_ticks = (long long)(microseconds * b->clock_khz / 1000.0);
+  b->version++;
   return MCGPU_OK;
 }
 
@@ -672,6 +758,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
     return MCGPU_ERR_NOMEM;
   }
   BatchSlot& S = b->slots[s];
+  S = BatchSlot();
   S.used = true;
   S.off = off;
   S.lay = L;
@@ -679,9 +766,11 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   S.n_pidx = d->n_primary;
   S.lrc = p->energy_offset;
   S.spread = pt.spread;
+  S.group = choose_group(*d, p->neighbor, pt.max_len);
   char* base = b->arena.base() + off;
-  BoxView& v = S.box.bv;
-  v = BoxView{};
+  BatchBox& E = b->h_table[s];
+  E = BatchBox{};
+  BoxView& v = E.bv;
   v.x = (double*)(base + L.x);
   v.y = (double*)(base + L.y);
   v.z = (double*)(base + L.z);
@@ -711,60 +800,59 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   v.cells.cap = L.cap;
   v.verlet.stale = (int32_t*)(base + L.vstale);
   v.verlet.count = (int32_t*)(base + L.vcount);
-  S.box.rng = (Rng31*)(base + L.rng);
-  S.box.max_t = d->max_translate;
-  S.box.max_r = d->max_rotate;
-  S.box.precision = p->precision;
-  S.box.record = p->record_trace ? 1 : 0;
-  S.box.tr_mol = (int32_t*)(base + L.trm);
-  S.box.tr_de = (double*)(base + L.trd);
-  S.box.tr_acc = (int8_t*)(base + L.tra);
-  Rng31 g;
-  rng31_seed(g, p->seed);
+  E.rng = b->d_rng + s;
+  E.max_t = d->max_translate;
+  E.max_r = d->max_rotate;
+  E.precision = p->precision;
+  E.record = p->record_trace ? 1 : 0;
+  E.tr_mol = (int32_t*)(base + L.trm);
+  E.tr_de = (double*)(base + L.trd);
+  E.tr_acc = (int8_t*)(base + L.tra);
+  rng31_seed(b->h_rng[s], p->seed);
   const int32_t zero[3] = {0, 0, 0};
   struct Copy {
-    size_t off;
+    void* dst;
     const void* src;
     size_t bytes;
-  } copies[] = {{L.x, d->x, 8 * (size_t)d->n_sites},
-                {L.y, d->y, 8 * (size_t)d->n_sites},
-                {L.z, d->z, 8 * (size_t)d->n_sites},
-                {L.ms, d->mol_start, 4 * (size_t)d->n_molecules},
-                {L.ml, d->mol_len, 4 * (size_t)d->n_molecules},
-                {L.mps, d->mol_pidx_start, 4 * (size_t)d->n_molecules},
-                {L.mpc, d->mol_pidx_count, 4 * (size_t)d->n_molecules},
-                {L.mp, pt.mol_param.data(), 4 * (size_t)d->n_molecules},
-                {L.pidx, d->primary_index, 4 * (size_t)d->n_primary},
-                {L.p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
-                {L.p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()},
-                {L.covf, zero, 4},
-                {L.vstale, zero, 4},
-                {L.vcount, zero, 4},
-                {L.rng, &g, sizeof(Rng31)}};
+  } copies[] = {{base + L.x, d->x, 8 * (size_t)d->n_sites},
+                {base + L.y, d->y, 8 * (size_t)d->n_sites},
+                {base + L.z, d->z, 8 * (size_t)d->n_sites},
+                {base + L.ms, d->mol_start, 4 * (size_t)d->n_molecules},
+                {base + L.ml, d->mol_len, 4 * (size_t)d->n_molecules},
+                {base + L.mps, d->mol_pidx_start, 4 * (size_t)d->n_molecules},
+                {base + L.mpc, d->mol_pidx_count, 4 * (size_t)d->n_molecules},
+                {base + L.mp, pt.mol_param.data(), 4 * (size_t)d->n_molecules},
+                {base + L.pidx, d->primary_index, 4 * (size_t)d->n_primary},
+                {base + L.p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
+                {base + L.p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()},
+                {base + L.covf, zero, 4},
+                {base + L.vstale, zero, 4},
+                {base + L.vcount, zero, 4},
+                {b->d_rng + s, b->h_rng + s, sizeof(Rng31)}};
   for (const Copy& c : copies) {
-    if (hipMemcpyAsync(base + c.off, c.src, c.bytes, hipMemcpyHostToDevice, b->stream) !=
+    if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, b->stream) !=
         hipSuccess) {
       free_slot(b, s);
       set_error("mcgpu_batch_add: upload failed");
       return MCGPU_ERR_HIP;
     }
   }
-  // The host arrays are read by the copies above: finish before returning.
+  // The sources are the caller's (and this call's) host arrays: wait for
+  // the copies; the set-up kernels below stay asynchronous.
+  if (hipStreamSynchronize(b->stream) != hipSuccess) {
+    free_slot(b, s);
+    set_error("mcgpu_batch_add: upload failed");
+    return MCGPU_ERR_HIP;
+  }
   if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream)) ||
-      (p->neighbor == MCGPU_NEIGHBOR_CELL && (rc = slot_rebuild_cells(b, s, pt.spread)))) {
+      (p->neighbor == MCGPU_NEIGHBOR_CELL && (rc = slot_rebuild_cells(b, s))) ||
+      (rc = slot_energy_async(b, s, 0))) {
     free_slot(b, s);
     return rc;
   }
-  double e0 = 0.0;
-  if ((rc = slot_energy(b, s, &e0))) {
-    free_slot(b, s);
-    return rc;
-  }
-  S.start_energy = e0 + p->energy_offset;
   BoxCtl& c = b->h_ctl[s];
   c = BoxCtl{};
   c.steps_target = p->steps;
-  c.energy = S.start_energy;
   c.next_stop = (p->ensemble == MCGPU_NPT) ? std::min<int64_t>(p->steps, p->volume_interval)
                                            : p->steps;
   c.status = p->steps > 0 ? kRunning : kDone;
@@ -772,7 +860,10 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
     free_slot(b, s);
     return rc;
   }
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  hipLaunchKernelGGL(k_set_start_energy, dim3(1), dim3(64), 0, b->stream, b->d_ctl + s,
+                     (const double*)(base + L.total), p->energy_offset);
+  MCGPU_HIP_TRY(hipGetLastError());
+  b->stale_ctl = true;  // the device now holds the start energy
   b->version++;
   *slot_out = s;
   return MCGPU_OK;
@@ -785,9 +876,9 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
   }
   int rc = batch_sync(b);
   if (rc) return rc;
+  b->h_ctl[s] = BoxCtl{};
   b->h_ctl[s].status = kDone;
   if ((rc = upload_ctl(b, s))) return rc;
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   free_slot(b, s);
   return MCGPU_OK;
 }
@@ -796,36 +887,51 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   if (!b || sweeps < 1) return MCGPU_ERR_INVALID;
   int rc = batch_sync(b);
   if (rc) return rc;
-  int n = 0;
+  int n[2] = {0, 0};
   for (int s = 0; s < b->max_boxes; s++)
-    if (b->slots[s].used && b->h_ctl[s].status == kRunning) b->h_active[n++] = s;
-  if (n == 0) return MCGPU_OK;
+    if (b->slots[s].used && b->h_ctl[s].status == kRunning) {
+      const int g = b->slots[s].group;
+      b->h_active[g * b->max_boxes + n[g]++] = s;
+    }
+  if (n[0] + n[1] == 0) return MCGPU_OK;
   const bool same = b->gexec && b->graph_version == b->version &&
-                    b->graph_sweeps == sweeps && b->graph_active == n;
+                    b->graph_sweeps == sweeps && b->n_active[0] == n[0] &&
+                    b->n_active[1] == n[1];
   if (!same) {
     if (b->gexec) MCGPU_HIP_TRY(hipGraphExecDestroy(b->gexec));
     if (b->graph) MCGPU_HIP_TRY(hipGraphDestroy(b->graph));
     b->gexec = nullptr;
     b->graph = nullptr;
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active, b->h_active, sizeof(int32_t) * n,
-                                 hipMemcpyHostToDevice, b->stream));
-    // The persistent launch of one tick is captured once and replayed
-    // while the box set and sweep count are unchanged.
+    for (int g = 0; g < 2; g++)
+      if (n[g])
+        MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * b->max_boxes,
+                                     b->h_active + g * b->max_boxes, sizeof(int32_t) * n[g],
+                                     hipMemcpyHostToDevice, b->stream));
+    // One tick = the persistent launch of each work-group-size group, the
+    // two running side by side; captured once and replayed while the box
+    // set and sweep count are unchanged.
     MCGPU_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_batch<256>, dim3(n), dim3(256), 0, b->stream, b->d_table,
-                       b->d_ctl, b->d_active, sweeps);
+    hipEventRecord(b->fork, b->stream);
+    hipStreamWaitEvent(b->stream2, b->fork, 0);
+    if (n[0])
+      hipLaunchKernelGGL(k_batch<256>, dim3(n[0]), dim3(256), 0, b->stream, b->d_table,
+                         b->d_ctl, b->d_active, sweeps, b->budget_ticks);
+    if (n[1])
+      hipLaunchKernelGGL(k_batch<512>, dim3(n[1]), dim3(512), 0, b->stream2, b->d_table,
+                         b->d_ctl, b->d_active + b->max_boxes, sweeps, b->budget_ticks);
+    hipEventRecord(b->join, b->stream2);
+    hipStreamWaitEvent(b->stream, b->join, 0);
     MCGPU_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
     MCGPU_HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->graph_version = b->version;
     b->graph_sweeps = sweeps;
-    b->graph_active = n;
+    b->n_active[0] = n[0];
+    b->n_active[1] = n[1];
   }
   MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
   MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
-                               hipMemcpyDeviceToHost, b->stream));
-  b->n_active = n;
+  if ((rc = readback(b))) return rc;
   b->pending = true;
   b->timing_pending = true;
   if (sync) return batch_sync(b);
@@ -845,12 +951,12 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   out->volume_moves = b->slots[s].vol_moves;
   out->volume_accepted = b->slots[s].vol_acc;
   out->energy = c.energy;
-  out->start_energy = b->slots[s].start_energy;
+  out->start_energy = c.start_energy;
   out->work_candidates = c.work_cand;
   out->work_sites = c.work_sites + c.work_self;
-  for (int k = 0; k < 3; k++) out->box[k] = b->slots[s].box.bv.L[k];
+  for (int k = 0; k < 3; k++) out->box[k] = b->h_table[s].bv.L[k];
   out->done = c.status == kDone;
-  out->fault = b->slots[s].fault;
+  out->fault = c.fault ? c.fault : b->slots[s].fault;
   return MCGPU_OK;
 }
 
@@ -862,7 +968,7 @@ extern "C" int mcgpu_batch_download(mcgpu_batch* b, int32_t s, double* x,
   }
   int rc = batch_sync(b);
   if (rc) return rc;
-  const BoxView& v = b->slots[s].box.bv;
+  const BoxView& v = b->h_table[s].bv;
   const size_t nb = 8 * (size_t)v.n_site;
   MCGPU_HIP_TRY(hipMemcpyAsync(x, v.x, nb, hipMemcpyDeviceToHost, b->stream));
   MCGPU_HIP_TRY(hipMemcpyAsync(y, v.y, nb, hipMemcpyDeviceToHost, b->stream));
@@ -880,16 +986,16 @@ extern "C" int mcgpu_batch_trace(mcgpu_batch* b, int32_t s, int32_t* mol,
   }
   int rc = batch_sync(b);
   if (rc) return rc;
-  const BatchSlot& S = b->slots[s];
-  if (!S.box.record) {
+  const BatchBox& E = b->h_table[s];
+  if (!E.record) {
     set_error("mcgpu_batch_trace: box was added without record_trace");
     return MCGPU_ERR_STATE;
   }
   const int64_t k = std::min<int64_t>(cap, b->h_ctl[s].steps_done);
   if (k > 0) {
-    if (mol) MCGPU_HIP_TRY(hipMemcpyAsync(mol, S.box.tr_mol, 4 * k, hipMemcpyDeviceToHost, b->stream));
-    if (delta) MCGPU_HIP_TRY(hipMemcpyAsync(delta, S.box.tr_de, 8 * k, hipMemcpyDeviceToHost, b->stream));
-    if (acc) MCGPU_HIP_TRY(hipMemcpyAsync(acc, S.box.tr_acc, k, hipMemcpyDeviceToHost, b->stream));
+    if (mol) MCGPU_HIP_TRY(hipMemcpyAsync(mol, E.tr_mol, 4 * k, hipMemcpyDeviceToHost, b->stream));
+    if (delta) MCGPU_HIP_TRY(hipMemcpyAsync(delta, E.tr_de, 8 * k, hipMemcpyDeviceToHost, b->stream));
+    if (acc) MCGPU_HIP_TRY(hipMemcpyAsync(acc, E.tr_acc, k, hipMemcpyDeviceToHost, b->stream));
     MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   }
   *n = k;
@@ -905,5 +1011,21 @@ extern "C" int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms,
   *avg_ms = b->kernel_launches ? b->kernel_ms_sum / b->kernel_launches : 0.0;
   b->kernel_ms_sum = 0.0;
   b->kernel_launches = 0;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_memory(mcgpu_batch* b, size_t* used, size_t* peak,
+                                  size_t* capacity) {
+  if (!b) return MCGPU_ERR_INVALID;
+  if (used) *used = b->arena.used();
+  if (peak) *peak = b->arena.peak();
+  if (capacity) *capacity = b->arena.capacity();
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_group(mcgpu_batch* b, int32_t s, int32_t* threads) {
+  if (!b || !threads || s < 0 || s >= b->max_boxes || !b->slots[s].used)
+    return MCGPU_ERR_RANGE;
+  *threads = kGroupNT[b->slots[s].group];
   return MCGPU_OK;
 }

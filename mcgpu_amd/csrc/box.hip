@@ -16,6 +16,7 @@
 
 #include "box_internal.h"
 #include "eval.h"
+#include "fastmath.h"
 
 namespace mcgpu {
 
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(NT) void k_move(BoxView bv, MoveArgs a,
   if (t == 0) {
     const double de = en - eo;
     int acc = 0;
-    if (a.metropolis) acc = (de < 0) || (exp(-(de) / bv.kT) >= a.u);
+    if (a.metropolis) acc = (de < 0) || (exp_fp64(-(de) / bv.kT) >= a.u);
     out->e_old = eo;
     out->e_new = en;
     out->de = de;
@@ -219,6 +220,16 @@ __global__ void k_volume_scale(BoxView bv, double f) {
     bv.x[s] += sx;
     bv.y[s] += sy;
     bv.z[s] += sz;
+  }
+}
+
+// Test hook: the device sin/cos/exp used by trial moves and Metropolis.
+__global__ void k_debug_math(const double* x, int n, double* s, double* c,
+                             double* e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    sincos_fp64(x[i], s[i], c[i]);
+    e[i] = exp_fp64(x[i]);
   }
 }
 
@@ -874,5 +885,28 @@ extern "C" int mcgpu_volume_scale(mcgpu_box* b, double factor) {
 extern "C" int mcgpu_box_size(mcgpu_box* b, double* edges3) {
   if (!b || !edges3) return MCGPU_ERR_INVALID;
   for (int k = 0; k < 3; k++) edges3[k] = b->bv.L[k];
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_debug_math(const double* x, int32_t n, double* s, double* c,
+                                double* e) {
+  if (!x || !s || !c || !e || n < 0) return MCGPU_ERR_INVALID;
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (n == 0) return MCGPU_OK;
+  double* d = nullptr;
+  MCGPU_HIP_TRY(hipMalloc(&d, sizeof(double) * 4 * (size_t)n));
+  hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, 0, d, n,
+                     d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
+  hipError_t err = hipDeviceSynchronize();
+  if (err == hipSuccess) err = hipMemcpy(s, d + n, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(c, d + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(e, d + 3 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) {
+    set_error(std::string("mcgpu_debug_math: ") + hipGetErrorString(err));
+    return MCGPU_ERR_HIP;
+  }
   return MCGPU_OK;
 }

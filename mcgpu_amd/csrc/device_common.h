@@ -55,6 +55,11 @@ struct VerletList {
 #define MCGPU_NB_CELL 1
 #define MCGPU_NB_VERLET 2
 
+// Fault codes a kernel records (and then stops the box) when a bound that
+// the data should always satisfy does not hold.
+#define MCGPU_FAULT_CANDIDATES 1  // neighbour source claims more candidates than molecules
+#define MCGPU_FAULT_MOLLEN 2      // molecule length outside [1, kMaxSites]
+
 struct BoxView {
   double* x;
   double* y;
@@ -90,19 +95,32 @@ __device__ __forceinline__ double wrap1(double d, double L) {
   return d;
 }
 
+// 1/sqrt(x) in FP64: the hardware estimate (v_rsq_f64) refined by two
+// Newton steps, y <- y (3/2 - x y^2 / 2); within ~2 ulp of the correctly
+// rounded value, at a fraction of the cost of a division plus a square root.
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
 // FP64 LJ + Coulomb of one site pair from the wrapped displacement
-// (SimulationStep.cpp:344-367; BruteForceStep.cpp:64-73).
+// (SimulationStep.cpp:344-367; BruteForceStep.cpp:64-73).  Callers skip
+// dummy sites; r^2 == 0 contributes nothing.
 __device__ __forceinline__ double pair_energy64(double dx, double dy, double dz,
                                                 const SiteParam64& a,
                                                 const SiteParam64& b) {
   const double r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 == 0.0 || a.real == 0.0 || b.real == 0.0) return 0.0;
-  const double inv_r2 = 1.0 / r2;
+  if (r2 == 0.0) return 0.0;
+  const double rinv = rsqrt64(r2);
+  const double inv_r2 = rinv * rinv;
   const double sig = a.ss * b.ss;
   const double s2 = sig * sig * inv_r2;
   const double s6 = s2 * s2 * s2;
   const double lj = 4.0 * (a.se * b.se) * (s6 * s6 - s6);
-  const double coul = (a.q * b.q * kCoulombK) * sqrt(inv_r2);
+  const double coul = (a.q * b.q * kCoulombK) * rinv;
   return lj + coul;
 }
 
@@ -119,7 +137,7 @@ __device__ __forceinline__ double pair_energy32(double dx, double dy, double dz,
                                                 const SiteParam64* b64) {
   const double r2d = dx * dx + dy * dy + dz * dz;
   const float r2 = (float)r2d;
-  if (r2 == 0.0f || a.real == 0.0f || b.real == 0.0f) return 0.0;
+  if (r2 == 0.0f) return 0.0;
   float rinv = __builtin_amdgcn_rsqf(r2);
   rinv = rinv * (1.5f - 0.5f * r2 * rinv * rinv);  // one Newton step
   const float inv_r2 = rinv * rinv;
@@ -152,15 +170,23 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sum over the lanes of a wave of per-wave partials held in LDS (lane w
+// reads partial w), in a fixed order; every lane gets the total.
+template <int NW, class T>
+__device__ __forceinline__ T wave_sum_partials(const T* lds) {
+  T v = lane_id() < NW ? lds[lane_id()] : T(0);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Block sum in a fixed order (deterministic for a given NT).
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
   if (lane_id() == 0) lds[wave_id()] = v;
   __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int w = 0; w < NT / 64; w++) t += lds[w];
+  const double t = wave_sum_partials<NT / 64>(lds);
   __syncthreads();
   return t;
 }
@@ -173,16 +199,16 @@ __device__ __forceinline__ int block_scan_flag(bool f, int* lds, int& total) {
   const int below = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) lds[wave_id()] = __popcll(m);
   __syncthreads();
-  int off = 0, tot = 0;
+  int v = lane < NT / 64 ? lds[lane] : 0;  // inclusive scan of wave counts
 #pragma unroll
-  for (int w = 0; w < NT / 64; w++) {
-    const int c = lds[w];
-    off += (w < wave_id()) ? c : 0;
-    tot += c;
+  for (int o = 1; o < NT / 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
   }
+  const int incl_prev = __shfl(v, wave_id() > 0 ? wave_id() - 1 : 0);
+  total = __shfl(v, NT / 64 - 1);
   __syncthreads();
-  total = tot;
-  return off + below;
+  return (wave_id() > 0 ? incl_prev : 0) + below;
 }
 
 // ------------------------------------------------------------ cell grid
@@ -220,18 +246,29 @@ struct CellRange {
   }
 };
 
+// Cells around an old cell co and a proposed cell cn.  Per dimension the
+// proposed cell is taken at its periodic (minimum-image) offset from co, so
+// a molecule crossing the box boundary spans {co-1 .. cn+1} across the wrap
+// (4 cells) rather than the whole row; when the span covers the row the row
+// is taken once.  A span over 4 cells (a jump of 2+ cells) cannot occur for
+// moves shorter than a cell; it is reported as a total over 64, and the
+// caller then scans every molecule.
 __device__ __forceinline__ CellRange make_range(const CellGrid& g,
                                                 const int co[3],
                                                 const int cn[3]) {
   CellRange r;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const int lo = (co[d] < cn[d] ? co[d] : cn[d]) - 1;
-    const int hi = (co[d] > cn[d] ? co[d] : cn[d]) + 1;
+    const int n = g.n[d];
+    int delta = cn[d] - co[d];
+    if (2 * delta > n) delta -= n;
+    if (2 * delta < -n) delta += n;
+    const int lo = co[d] + (delta < 0 ? delta : 0) - 1;
+    const int hi = co[d] + (delta > 0 ? delta : 0) + 1;
     const int c = hi - lo + 1;
-    if (c >= g.n[d]) {
+    if (c >= n) {
       r.lo[d] = 0;
-      r.cnt[d] = g.n[d];
+      r.cnt[d] = n;
     } else {
       r.lo[d] = lo;
       r.cnt[d] = c;
@@ -239,5 +276,8 @@ __device__ __forceinline__ CellRange make_range(const CellGrid& g,
   }
   return r;
 }
+
+// CellRange::cell wraps each coordinate once, which is exact because
+// lo >= -n and lo + cnt - 1 < 2 n for every range make_range builds.
 
 }  // namespace mcgpu

@@ -1,15 +1,19 @@
 // Work-group level evaluation of one molecule's intermolecular energy at its
 // current and (optionally) proposed position, in one pass over candidates.
 //
-// Phase 1 (candidate scan): every lane takes one candidate molecule j from
-// the neighbour source (all molecules, the cell range around the old and new
-// cells, or the Verlet list), tests the primary-site cutoff for the old and
-// new position (SimulationStep.cpp:317-330), and the hits are compacted into
-// LDS in candidate order by a ballot/popcount block scan.
-// Phase 2 (pair sum): lanes take (hit, site of j) items and loop over the
-// moving molecule's sites held in LDS, accumulating FP64 sums
-// (BruteForceStep.cpp:48-79).  A fixed-order block reduction follows, so the
-// result is deterministic for a given work-group size.
+// Phase 1 (candidate scan): lanes take candidate molecules j from the
+// neighbour source -- every molecule, the occupied slots of the cells around
+// the old and proposed cells (enumerated through a prefix over the cell
+// counts, so no lane looks at an empty slot), or the Verlet list -- test the
+// primary-site cutoff at the old and proposed position (SimulationStep.cpp:
+// 317-330), and the hits are compacted into LDS in candidate order by a
+// ballot/popcount block scan.
+// Phase 2 (pair sum): lanes are grouped Aw = next_pow2(m_i) at a time; a
+// group takes one (hit, site b) item and each lane of it one site a of the
+// moving molecule, so every (a, b) pair of every hit is one lane-step
+// (BruteForceStep.cpp:48-79).  Sums are FP64; one fixed-order block
+// reduction ends the evaluation, so results are deterministic for a given
+// work-group size.
 #pragma once
 
 #include "device_common.h"
@@ -26,6 +30,8 @@ struct EvalShared {
   int32_t pofs[kMaxSites];  // primary offsets within the molecule
   int32_t cand_j[kCap];
   int32_t cand_f[kCap];  // bit0: in range at old position, bit1: at new
+  int32_t cell_id[64];   // cells of the current range (at most 4x4x4)
+  int32_t cell_pre[65];  // prefix of their occupancies
   double red[NT / 64];
   double red2[NT / 64];
   unsigned long long redc[NT / 64];
@@ -33,7 +39,9 @@ struct EvalShared {
   int32_t npri;
   int32_t len;
   int32_t nreal;
+  int32_t aw_log2;  // log2 of the lane-group width of the pair phase
   int32_t flag;
+  int32_t fault;    // MCGPU_FAULT_* raised by the last evaluation (0 = none)
 };
 
 // Loads molecule `mol`'s current sites (and parameters) into LDS.
@@ -42,7 +50,8 @@ __device__ __forceinline__ void load_molecule(const BoxView& bv, int mol,
                                               EvalShared<NT>& sh) {
   const int t = threadIdx.x;
   const int start = bv.mol_start[mol];
-  const int len = bv.mol_len[mol];
+  int len = bv.mol_len[mol];
+  len = len < 1 ? 1 : (len > kMaxSites ? kMaxSites : len);
   const int pbase = bv.mol_param[mol];
   for (int k = t; k < len; k += NT) {
     sh.ox[k] = bv.x[start + k];
@@ -52,13 +61,19 @@ __device__ __forceinline__ void load_molecule(const BoxView& bv, int mol,
     sh.q32[k] = bv.p32[pbase + k];
   }
   if (t == 0) {
-    const int ps = bv.mol_pidx_start[mol], pc = bv.mol_pidx_count[mol];
+    const int ps = bv.mol_pidx_start[mol];
+    int pc = bv.mol_pidx_count[mol];
+    pc = pc < 1 ? 1 : (pc > kMaxSites ? kMaxSites : pc);
+    sh.fault = (bv.mol_len[mol] != len) ? MCGPU_FAULT_MOLLEN : 0;
     for (int k = 0; k < pc; k++) sh.pofs[k] = bv.pidx[ps + k] - start;
     sh.npri = pc;
     sh.len = len;
     int nr = 0;
     for (int k = 0; k < len; k++) nr += bv.p64[pbase + k].real != 0.0;
     sh.nreal = nr;
+    int lg = 0;
+    while ((1 << lg) < len) lg++;
+    sh.aw_log2 = lg;
   }
 }
 
@@ -92,13 +107,18 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
                                            unsigned& nsites) {
   const int M = bv.max_len;
   const int len = sh.len;
-  for (int item = threadIdx.x; item < nhits * M; item += NT) {
+  const int lg = sh.aw_log2;
+  const int a = threadIdx.x & ((1 << lg) - 1);
+  const int G = NT >> lg;
+  const bool lane_real = a < len && sh.q64[a < len ? a : 0].real != 0.0;
+  for (int item = threadIdx.x >> lg; item < nhits * M; item += G) {
     const int k = item / M;
     const int b = item - k * M;
     const int j = sh.cand_j[k];
     if (b >= bv.mol_len[j]) continue;
+    nsites += (a == 0);
+    if (!lane_real) continue;
     const int f = sh.cand_f[k];
-    nsites++;
     const int site = bv.mol_start[j] + b;
     const int pi = bv.mol_param[j] + b;
     const double xb = bv.x[site], yb = bv.y[site], zb = bv.z[site];
@@ -106,47 +126,39 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
       const SiteParam32 pb = bv.p32[pi];
       if (pb.real == 0.0f) continue;
       if (f & 1) {
-        for (int a = 0; a < len; a++)
-          e_old += pair_energy32(wrap1(xb - sh.ox[a], bv.L[0]),
-                                 wrap1(yb - sh.oy[a], bv.L[1]),
-                                 wrap1(zb - sh.oz[a], bv.L[2]), sh.q32[a], pb,
-                                 sh.q64[a], bv.p64 + pi);
-        npairs += sh.nreal;
+        e_old += pair_energy32(wrap1(xb - sh.ox[a], bv.L[0]),
+                               wrap1(yb - sh.oy[a], bv.L[1]),
+                               wrap1(zb - sh.oz[a], bv.L[2]), sh.q32[a], pb,
+                               sh.q64[a], bv.p64 + pi);
+        npairs++;
       }
-      if (f & 2) {
-        for (int a = 0; a < len; a++)
-          e_new += pair_energy32(wrap1(xb - sh.nx[a], bv.L[0]),
-                                 wrap1(yb - sh.ny[a], bv.L[1]),
-                                 wrap1(zb - sh.nz[a], bv.L[2]), sh.q32[a], pb,
-                                 sh.q64[a], bv.p64 + pi);
-      }
+      if (f & 2)
+        e_new += pair_energy32(wrap1(xb - sh.nx[a], bv.L[0]),
+                               wrap1(yb - sh.ny[a], bv.L[1]),
+                               wrap1(zb - sh.nz[a], bv.L[2]), sh.q32[a], pb,
+                               sh.q64[a], bv.p64 + pi);
     } else {
       const SiteParam64 pb = bv.p64[pi];
       if (pb.real == 0.0) continue;
       if (f & 1) {
-        for (int a = 0; a < len; a++)
-          e_old += pair_energy64(wrap1(xb - sh.ox[a], bv.L[0]),
-                                 wrap1(yb - sh.oy[a], bv.L[1]),
-                                 wrap1(zb - sh.oz[a], bv.L[2]), sh.q64[a], pb);
-        npairs += sh.nreal;
+        e_old += pair_energy64(wrap1(xb - sh.ox[a], bv.L[0]),
+                               wrap1(yb - sh.oy[a], bv.L[1]),
+                               wrap1(zb - sh.oz[a], bv.L[2]), sh.q64[a], pb);
+        npairs++;
       }
-      if (f & 2) {
-        for (int a = 0; a < len; a++)
-          e_new += pair_energy64(wrap1(xb - sh.nx[a], bv.L[0]),
-                                 wrap1(yb - sh.ny[a], bv.L[1]),
-                                 wrap1(zb - sh.nz[a], bv.L[2]), sh.q64[a], pb);
-      }
+      if (f & 2)
+        e_new += pair_energy64(wrap1(xb - sh.nx[a], bv.L[0]),
+                               wrap1(yb - sh.ny[a], bv.L[1]),
+                               wrap1(zb - sh.nz[a], bv.L[2]), sh.q64[a], pb);
     }
   }
   __syncthreads();
 }
 
 // Neighbour source actually usable for this evaluation (block-uniform).
-__device__ __forceinline__ int pick_source(const BoxView& bv, const double* ox,
-                                           const double* oy, const double* oz,
-                                           const double* nx, const double* ny,
-                                           const double* nz, int p0, int mol,
-                                           bool has_new) {
+__device__ __forceinline__ int pick_source(const BoxView& bv, const double* nx,
+                                           const double* ny, const double* nz,
+                                           int p0, int mol, bool has_new) {
   if (bv.mode == MCGPU_NB_CELL && bv.cells.overflow[0] == 0) return MCGPU_NB_CELL;
   if (bv.mode == MCGPU_NB_VERLET && bv.verlet.stale[0] == 0 &&
       bv.verlet.count[mol] <= bv.verlet.cap) {
@@ -162,10 +174,6 @@ __device__ __forceinline__ int pick_source(const BoxView& bv, const double* ox,
   return MCGPU_NB_BRUTE;
 }
 
-// Evaluates E_mol over candidates j >= jlo (j != mol) at the LDS-held old
-// position, and at the proposed position when has_new.  Must be called by
-// all NT threads after load_molecule (+ proposed coords) and a barrier.
-// Results are block-uniform.
 // Work counters of one evaluation (block-uniform): candidate molecules
 // examined and sites of in-range molecules loaded; they give the kernel's
 // algorithmic bytes (24 B per candidate primary, 24 B per site).
@@ -174,6 +182,10 @@ struct EvalWork {
   unsigned sites;
 };
 
+// Evaluates E_mol over candidates j >= jlo (j != mol) at the LDS-held old
+// position, and at the proposed position when has_new.  Must be called by
+// all NT threads after load_molecule (+ proposed coords) and a barrier.
+// Results are block-uniform.
 template <int NT, bool MIXED>
 __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
                               bool has_new, EvalShared<NT>& sh, double& e_old,
@@ -181,31 +193,59 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
                               EvalWork* work = nullptr) {
   const int t = threadIdx.x;
   const int p0 = sh.pofs[0];
-  const int src = pick_source(bv, sh.ox, sh.oy, sh.oz, sh.nx, sh.ny, sh.nz, p0,
-                              mol, has_new);
+  int src = pick_source(bv, sh.nx, sh.ny, sh.nz, p0, mol, has_new);
+  int nitems, ncells = 0;
   CellRange range{};
-  int nitems;
-  const int cap = bv.cells.cap;
   if (src == MCGPU_NB_CELL) {
+    const CellGrid& g = bv.cells;
     int co[3], cn[3];
-    co[0] = cell_coord(sh.ox[p0], bv.cells.inv_w[0], bv.cells.n[0]);
-    co[1] = cell_coord(sh.oy[p0], bv.cells.inv_w[1], bv.cells.n[1]);
-    co[2] = cell_coord(sh.oz[p0], bv.cells.inv_w[2], bv.cells.n[2]);
+    co[0] = cell_coord(sh.ox[p0], g.inv_w[0], g.n[0]);
+    co[1] = cell_coord(sh.oy[p0], g.inv_w[1], g.n[1]);
+    co[2] = cell_coord(sh.oz[p0], g.inv_w[2], g.n[2]);
     if (has_new) {
-      cn[0] = cell_coord(sh.nx[p0], bv.cells.inv_w[0], bv.cells.n[0]);
-      cn[1] = cell_coord(sh.ny[p0], bv.cells.inv_w[1], bv.cells.n[1]);
-      cn[2] = cell_coord(sh.nz[p0], bv.cells.inv_w[2], bv.cells.n[2]);
+      cn[0] = cell_coord(sh.nx[p0], g.inv_w[0], g.n[0]);
+      cn[1] = cell_coord(sh.ny[p0], g.inv_w[1], g.n[1]);
+      cn[2] = cell_coord(sh.nz[p0], g.inv_w[2], g.n[2]);
     } else {
       cn[0] = co[0];
       cn[1] = co[1];
       cn[2] = co[2];
     }
-    range = make_range(bv.cells, co, cn);
-    nitems = range.total() * cap;
+    range = make_range(g, co, cn);
+    ncells = range.total();
+    if (ncells > 64) src = MCGPU_NB_BRUTE;  // see make_range (uniform)
+  }
+  if (src == MCGPU_NB_CELL) {
+    const CellGrid& g = bv.cells;
+    // ncells <= 64: one wave builds the prefix of the cell occupancies
+    if (t < 64) {
+      int n = 0;
+      if (t < ncells) {
+        const int c = range.cell(g, t);
+        n = g.count[c];
+        n = n < g.cap ? n : g.cap;
+        sh.cell_id[t] = c;
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(n, o);
+        if (t >= o) n += u;
+      }
+      sh.cell_pre[t + 1] = n;
+      if (t == 0) sh.cell_pre[0] = 0;
+    }
+    __syncthreads();
+    nitems = sh.cell_pre[ncells];
   } else if (src == MCGPU_NB_VERLET) {
     nitems = bv.verlet.count[mol];
   } else {
     nitems = bv.n_mol - jlo;
+  }
+  // A neighbour source never holds more candidates than molecules; if it
+  // claims so its lists are corrupt: record it and evaluate nothing.
+  if (nitems < 0 || nitems > bv.n_mol) {
+    if (t == 0) sh.fault = MCGPU_FAULT_CANDIDATES;
+    nitems = 0;
   }
 
   double eo = 0.0, en = 0.0;
@@ -217,10 +257,12 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
     int j = -1;
     if (item < nitems) {
       if (src == MCGPU_NB_CELL) {
-        const int k = item / cap;
-        const int s = item - k * cap;
-        const int c = range.cell(bv.cells, k);
-        j = (s < bv.cells.count[c]) ? bv.cells.slots[c * cap + s] : -1;
+        int lo = 0, hi = ncells;  // last cell whose prefix <= item
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (sh.cell_pre[mid] <= item) lo = mid; else hi = mid;
+        }
+        j = bv.cells.slots[sh.cell_id[lo] * bv.cells.cap + (item - sh.cell_pre[lo])];
       } else if (src == MCGPU_NB_VERLET) {
         j = bv.verlet.list[mol * bv.verlet.cap + item];
       } else {
@@ -262,14 +304,9 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
     sh.redc[wave_id()] = wc;
   }
   __syncthreads();
-  double so = 0.0, sn = 0.0;
-  unsigned long long sc = 0;
-#pragma unroll
-  for (int w = 0; w < NT / 64; w++) {
-    so += sh.red[w];
-    sn += sh.red2[w];
-    sc += sh.redc[w];
-  }
+  const double so = wave_sum_partials<NT / 64>(sh.red);
+  const double sn = wave_sum_partials<NT / 64>(sh.red2);
+  const unsigned long long sc = wave_sum_partials<NT / 64>(sh.redc);
   __syncthreads();
   e_old = so;
   e_new = has_new ? sn : 0.0;

@@ -9,8 +9,11 @@
 // retirement.  Latency of a job = finish time - arrival time.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 #include <string>
 #include <thread>
 #include <vector>
@@ -53,7 +56,10 @@ extern "C" int mcgpu_server_create(const mcgpu_server_config* cfg,
   if (!s) return MCGPU_ERR_NOMEM;
   s->cfg = *cfg;
   int rc = mcgpu_batch_create(cfg->max_boxes, cfg->hbm_budget_bytes, &s->batch);
+  if (rc == MCGPU_OK && cfg->tick_budget_us > 0)
+    rc = mcgpu_batch_set_tick_budget(s->batch, cfg->tick_budget_us);
   if (rc != MCGPU_OK) {
+    if (s->batch) mcgpu_batch_destroy(s->batch);
     delete s;
     return rc;
   }
@@ -102,6 +108,14 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
   size_t next = 0, completed = 0;
   int64_t peak = 0;
   mcgpu_box_stats st;
+  // MCGPU_SERVER_LOG=<seconds>: a progress line on stderr at that period
+  const char* log_env = std::getenv("MCGPU_SERVER_LOG");
+  const double log_every = log_env ? std::atof(log_env) : 0.0;
+  double next_log = log_every;
+  // No-progress watchdog: with jobs resident or already arrived, a tick or an
+  // admission must complete within this many seconds.
+  const double kStallLimit = 60.0;
+  double last_progress = 0.0;
   while (completed < s->jobs.size()) {
     // admit every arrived job that fits (FIFO; the head waits for memory)
     const double t = now();
@@ -109,7 +123,13 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       Job& j = s->jobs[order[next]];
       const mcgpu_box_desc d = mcgpu::hostbox_ref(j.box).desc();
       int32_t slot = -1;
+      const double ta = now();
       const int rc = mcgpu_batch_add(s->batch, &d, &j.params, &slot);
+      if (log_every > 0 && now() - ta > log_every) {
+        std::fprintf(stderr, "[server] slow admission: job %lld (%d molecules) took %.2fs rc=%d\n",
+                     (long long)j.res.job_id, d.n_molecules, now() - ta, rc);
+        std::fflush(stderr);
+      }
       if (rc == MCGPU_ERR_NOMEM && !resident.empty()) break;  // wait for room
       next++;
       if (rc != MCGPU_OK) {  // failed job: isolated, reported, not retried
@@ -120,10 +140,19 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       }
       j.slot = slot;
       j.res.admit_s = now();
+      last_progress = j.res.admit_s;
       resident.push_back((int32_t)order[next - 1]);
     }
     peak = std::max<int64_t>(peak, (int64_t)resident.size());
+    const bool arrived = next < order.size() && s->jobs[order[next]].res.arrival_s <= now();
+    if ((arrived || !resident.empty()) && now() - last_progress > kStallLimit) {
+      mcgpu::set_error("mcgpu_server_run: no progress for 60 s (" +
+                       std::to_string(resident.size()) + " resident, " +
+                       std::to_string(completed) + " done)");
+      return MCGPU_ERR_STATE;
+    }
     if (resident.empty()) {
+      if (!arrived) last_progress = now();
       if (next < order.size()) {
         const double wait = s->jobs[order[next]].res.arrival_s - now();
         if (wait > 0)
@@ -131,9 +160,23 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       }
       continue;
     }
+    const double tt = now();
     int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 1);
     if (rc != MCGPU_OK) return rc;
     s->stats.ticks++;
+    last_progress = now();
+    if (log_every > 0 && now() - tt > log_every) {
+      std::fprintf(stderr, "[server] slow tick %lld: %.2fs with %zu resident\n",
+                   (long long)s->stats.ticks, now() - tt, resident.size());
+      std::fflush(stderr);
+    }
+    if (log_every > 0 && now() >= next_log) {
+      std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu admitted=%zu done=%zu moves=%lld\n",
+                   now(), (long long)s->stats.ticks, resident.size(), next, completed,
+                   (long long)s->stats.total_moves);
+      std::fflush(stderr);
+      next_log = now() + log_every;
+    }
     // retire finished jobs
     for (size_t k = 0; k < resident.size();) {
       Job& j = s->jobs[resident[k]];
@@ -146,6 +189,11 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
         j.res.accepted = st.accepted;
         j.res.steps = st.steps_done;
         j.res.status = st.fault ? MCGPU_ERR_STATE : MCGPU_OK;
+        if (st.fault && log_every > 0) {
+          std::fprintf(stderr, "[server] job %lld stopped by device fault %d after %lld moves\n",
+                       (long long)j.res.job_id, st.fault, (long long)st.steps_done);
+          std::fflush(stderr);
+        }
         s->stats.total_moves += st.steps_done;
         s->stats.work_candidates += st.work_candidates;
         s->stats.work_sites += st.work_sites;
@@ -162,6 +210,8 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
   }
   s->stats.wall_s = now();
   s->stats.peak_resident = peak;
+  size_t used = 0, pk = 0, capb = 0;
+  if (mcgpu_batch_memory(s->batch, &used, &pk, &capb) == MCGPU_OK) s->stats.peak_bytes = pk;
   double avg_ms = 0;
   int64_t launches = 0;
   if (mcgpu_batch_kernel_ms(s->batch, &avg_ms, &launches) == MCGPU_OK)

@@ -143,3 +143,22 @@ def test_batch_admission_by_footprint():
     s3 = b.add(hb, steps=1000)
     b.run(100)
     assert b.stats(s3).steps_done == 100 and b.stats(s2).steps_done == 100
+
+
+def test_batch_cells_equal_brute_over_boundary_crossings():
+    """A box with 8 cells per dimension run long enough that many moves
+    carry a molecule's primary across the periodic boundary (the cell range
+    then spans the wrap): cell-list and all-molecule traces must agree and
+    no box may stop on a device fault."""
+    hb = m.HostBox.generate(m.BICYCLIC, 7000, density=1.10, cutoff=12.0, seed=91,
+                            mc_seed=17)
+    steps = 20000
+    b = m.Batch(4)
+    s_cell = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+    s_brute = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_BRUTE, record_trace=True)
+    st = run_all(b, [s_cell, s_brute], steps, 5000)
+    assert all(x.fault == 0 for x in st)
+    tc, tb = b.trace(s_cell), b.trace(s_brute)
+    assert np.array_equal(tc.mol, tb.mol)
+    assert np.array_equal(tc.accepted, tb.accepted)
+    assert de_ok(tc.de, tb.de)
