@@ -313,7 +313,7 @@ class Arena {
 
 // Offsets of one box's arrays inside its arena block.
 struct BoxLayout {
-  size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, pidx, p64, p32;
+  size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, meta, pidx, p64, p32;
   size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, partial, pairs,
       total, bx, by, bz, trm, trd, tra;
   size_t bytes;
@@ -341,6 +341,7 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   L.mps = take(4 * nm);
   L.mpc = take(4 * nm);
   L.mp = take(4 * nm);
+  L.meta = take(sizeof(MolMeta) * nm);
   L.pidx = take(4 * np);
   L.p64 = take(sizeof(SiteParam64) * pt.p64.size());
   L.p32 = take(sizeof(SiteParam32) * pt.p32.size());
@@ -782,6 +783,8 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   v.mol_pidx_start = (int32_t*)(base + L.mps);
   v.mol_pidx_count = (int32_t*)(base + L.mpc);
   v.mol_param = (int32_t*)(base + L.mp);
+  v.meta = (MolMeta*)(base + L.meta);
+  v.single_primary = pt.single_primary;
   v.pidx = (int32_t*)(base + L.pidx);
   v.p64 = (SiteParam64*)(base + L.p64);
   v.p32 = (SiteParam32*)(base + L.p32);
@@ -822,6 +825,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
                 {base + L.mps, d->mol_pidx_start, 4 * (size_t)d->n_molecules},
                 {base + L.mpc, d->mol_pidx_count, 4 * (size_t)d->n_molecules},
                 {base + L.mp, pt.mol_param.data(), 4 * (size_t)d->n_molecules},
+                {base + L.meta, pt.meta.data(), sizeof(MolMeta) * (size_t)d->n_molecules},
                 {base + L.pidx, d->primary_index, 4 * (size_t)d->n_primary},
                 {base + L.p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
                 {base + L.p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()},

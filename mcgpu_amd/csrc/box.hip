@@ -296,6 +296,7 @@ int ensure_device() {
 ParamTable build_param_table(const mcgpu_box_desc& d) {
   ParamTable t;
   t.mol_param.resize(d.n_molecules);
+  t.meta.resize(d.n_molecules);
   std::map<std::vector<uint64_t>, int32_t> sig;
   for (int32_t m = 0; m < d.n_molecules; m++) {
     const int32_t s0 = d.mol_start[m], len = d.mol_len[m];
@@ -329,6 +330,8 @@ ParamTable build_param_table(const mcgpu_box_desc& d) {
       it = sig.emplace(key, base).first;
     }
     t.mol_param[m] = it->second;
+    t.meta[m] = MolMeta{s0, len, it->second, d.mol_pidx_start[m]};
+    if (d.mol_pidx_count[m] != 1 || d.mol_pidx_start[m] != m) t.single_primary = 0;
     // primary spread (rigid molecules: fixed)
     const int32_t p0 = d.primary_index[d.mol_pidx_start[m]];
     for (int32_t k = 1; k < d.mol_pidx_count[m]; k++) {
@@ -601,6 +604,7 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   }
   double *x, *y, *z, *px, *py, *pz;
   int32_t *ms, *ml, *mps, *mpc, *mp, *pi;
+  MolMeta* mm;
   SiteParam64* p64;
   SiteParam32* p32;
   ALLOC(x, ns);
@@ -614,6 +618,7 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   ALLOC(mps, nm);
   ALLOC(mpc, nm);
   ALLOC(mp, nm);
+  ALLOC(mm, nm);
   ALLOC(pi, np);
   ALLOC(p64, pt.p64.size());
   ALLOC(p32, pt.p32.size());
@@ -639,6 +644,8 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   v.mol_pidx_start = mps;
   v.mol_pidx_count = mpc;
   v.mol_param = mp;
+  v.meta = mm;
+  v.single_primary = pt.single_primary;
   v.pidx = pi;
   v.p64 = p64;
   v.p32 = p32;
@@ -669,6 +676,7 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
                 {mps, d->mol_pidx_start, 4 * nm},
                 {mpc, d->mol_pidx_count, 4 * nm},
                 {mp, pt.mol_param.data(), 4 * nm},
+                {mm, pt.meta.data(), sizeof(MolMeta) * nm},
                 {pi, d->primary_index, 4 * np},
                 {p64, pt.p64.data(), sizeof(SiteParam64) * pt.p64.size()},
                 {p32, pt.p32.data(), sizeof(SiteParam32) * pt.p32.size()}};

@@ -39,6 +39,8 @@ struct ParamTable {
   std::vector<SiteParam64> p64;
   std::vector<SiteParam32> p32;
   std::vector<int32_t> mol_param;
+  std::vector<MolMeta> meta;
+  int32_t single_primary = 1;
   double spread = 0.0;  // max distance of a primary from the first primary
   int32_t max_len = 0;
 };

@@ -31,6 +31,13 @@ struct SiteParam32 {
   float ss, se, q, real;
 };
 
+// Per-molecule record read with one 16-byte load in the candidate scan:
+// first site, site count, first entry in the site-parameter table, first
+// entry in the primary list.
+struct alignas(16) MolMeta {
+  int32_t start, len, param, pstart;
+};
+
 struct CellGrid {
   int32_t n[3];       // cells per dimension (>= 1)
   int32_t cap;        // slots per cell
@@ -72,6 +79,7 @@ struct BoxView {
   const int32_t* mol_pidx_start;
   const int32_t* mol_pidx_count;
   const int32_t* mol_param;
+  const MolMeta* meta;  // [n_mol]
   const int32_t* pidx;
   const SiteParam64* p64;
   const SiteParam32* p32;
@@ -79,6 +87,8 @@ struct BoxView {
   int32_t n_site;
   int32_t max_len;
   int32_t mode;  // MCGPU_NB_*
+  int32_t single_primary;  // every molecule j has one primary, entry j
+  int32_t pad0;
   double L[3];
   double cutoff2;
   double kT;

@@ -22,14 +22,16 @@ namespace mcgpu {
 
 template <int NT>
 struct EvalShared {
-  static constexpr int kCap = NT * 4 < 256 ? 256 : NT * 4;
+  static constexpr int kCap = NT * 2 < 256 ? 256 : NT * 2;
   double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
   double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
   SiteParam64 q64[kMaxSites];
   SiteParam32 q32[kMaxSites];
   int32_t pofs[kMaxSites];  // primary offsets within the molecule
-  int32_t cand_j[kCap];
-  int32_t cand_f[kCap];  // bit0: in range at old position, bit1: at new
+  // in-range molecules of the current evaluation (hits), compacted:
+  int32_t cand_start[kCap];  // first site
+  int32_t cand_param[kCap];  // first site-parameter entry
+  int32_t cand_lf[kCap];     // site count | flags << 8 (bit0 old, bit1 new)
   int32_t cell_id[64];   // cells of the current range (at most 4x4x4)
   int32_t cell_pre[65];  // prefix of their occupancies
   double red[NT / 64];
@@ -77,13 +79,14 @@ __device__ __forceinline__ void load_molecule(const BoxView& bv, int mol,
   }
 }
 
-__device__ __forceinline__ bool primary_in_range(const BoxView& bv, int j,
+// Any primary pair of (moving molecule at X/Y/Z, molecule with primaries
+// ps .. ps+pc-1 in the mirror) within the cutoff (SimulationStep.cpp:317-330).
+__device__ __forceinline__ bool primary_in_range(const BoxView& bv, int ps, int pc,
                                                  const double* X,
                                                  const double* Y,
                                                  const double* Z,
                                                  const int32_t* pofs,
                                                  int npri) {
-  const int ps = bv.mol_pidx_start[j], pc = bv.mol_pidx_count[j];
   bool in = false;
   for (int q = 0; q < pc; q++) {
     const double qx = bv.px[ps + q], qy = bv.py[ps + q], qz = bv.pz[ps + q];
@@ -98,7 +101,9 @@ __device__ __forceinline__ bool primary_in_range(const BoxView& bv, int j,
   return in;
 }
 
-// Sums the pair energies of the compacted hits into e_old / e_new.
+// Sums the pair energies of the compacted hits into e_old / e_new.  Each
+// lane handles two (hit, site b) items per iteration so the site loads of
+// both are in flight together.
 template <int NT, bool MIXED>
 __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
                                            int nhits, double& e_old,
@@ -111,45 +116,80 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
   const int a = threadIdx.x & ((1 << lg) - 1);
   const int G = NT >> lg;
   const bool lane_real = a < len && sh.q64[a < len ? a : 0].real != 0.0;
-  for (int item = threadIdx.x >> lg; item < nhits * M; item += G) {
-    const int k = item / M;
-    const int b = item - k * M;
-    const int j = sh.cand_j[k];
-    if (b >= bv.mol_len[j]) continue;
-    nsites += (a == 0);
-    if (!lane_real) continue;
-    const int f = sh.cand_f[k];
-    const int site = bv.mol_start[j] + b;
-    const int pi = bv.mol_param[j] + b;
-    const double xb = bv.x[site], yb = bv.y[site], zb = bv.z[site];
+  const int total = nhits * M;
+  for (int i0 = threadIdx.x >> lg; i0 < total; i0 += 2 * G) {
+    int site[2], pi[2], f[2];
+    bool ok[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int item = i0 + u * G;
+      ok[u] = false;
+      site[u] = pi[u] = f[u] = 0;
+      if (item < total) {
+        const int k = item / M;
+        const int b = item - k * M;
+        const int lf = sh.cand_lf[k];
+        if (b < (lf & 0xff)) {
+          ok[u] = true;
+          site[u] = sh.cand_start[k] + b;
+          pi[u] = sh.cand_param[k] + b;
+          f[u] = lf >> 8;
+        }
+      }
+      nsites += (ok[u] && a == 0);
+      ok[u] = ok[u] && lane_real;
+    }
+    double xb[2], yb[2], zb[2];
     if (MIXED) {
-      const SiteParam32 pb = bv.p32[pi];
-      if (pb.real == 0.0f) continue;
-      if (f & 1) {
-        e_old += pair_energy32(wrap1(xb - sh.ox[a], bv.L[0]),
-                               wrap1(yb - sh.oy[a], bv.L[1]),
-                               wrap1(zb - sh.oz[a], bv.L[2]), sh.q32[a], pb,
-                               sh.q64[a], bv.p64 + pi);
-        npairs++;
+      SiteParam32 pb[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (ok[u]) {
+          xb[u] = bv.x[site[u]];
+          yb[u] = bv.y[site[u]];
+          zb[u] = bv.z[site[u]];
+          pb[u] = bv.p32[pi[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!ok[u] || pb[u].real == 0.0f) continue;
+        if (f[u] & 1) {
+          e_old += pair_energy32(wrap1(xb[u] - sh.ox[a], bv.L[0]),
+                                 wrap1(yb[u] - sh.oy[a], bv.L[1]),
+                                 wrap1(zb[u] - sh.oz[a], bv.L[2]), sh.q32[a], pb[u],
+                                 sh.q64[a], bv.p64 + pi[u]);
+          npairs++;
+        }
+        if (f[u] & 2)
+          e_new += pair_energy32(wrap1(xb[u] - sh.nx[a], bv.L[0]),
+                                 wrap1(yb[u] - sh.ny[a], bv.L[1]),
+                                 wrap1(zb[u] - sh.nz[a], bv.L[2]), sh.q32[a], pb[u],
+                                 sh.q64[a], bv.p64 + pi[u]);
       }
-      if (f & 2)
-        e_new += pair_energy32(wrap1(xb - sh.nx[a], bv.L[0]),
-                               wrap1(yb - sh.ny[a], bv.L[1]),
-                               wrap1(zb - sh.nz[a], bv.L[2]), sh.q32[a], pb,
-                               sh.q64[a], bv.p64 + pi);
     } else {
-      const SiteParam64 pb = bv.p64[pi];
-      if (pb.real == 0.0) continue;
-      if (f & 1) {
-        e_old += pair_energy64(wrap1(xb - sh.ox[a], bv.L[0]),
-                               wrap1(yb - sh.oy[a], bv.L[1]),
-                               wrap1(zb - sh.oz[a], bv.L[2]), sh.q64[a], pb);
-        npairs++;
+      SiteParam64 pb[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (ok[u]) {
+          xb[u] = bv.x[site[u]];
+          yb[u] = bv.y[site[u]];
+          zb[u] = bv.z[site[u]];
+          pb[u] = bv.p64[pi[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!ok[u] || pb[u].real == 0.0) continue;
+        if (f[u] & 1) {
+          e_old += pair_energy64(wrap1(xb[u] - sh.ox[a], bv.L[0]),
+                                 wrap1(yb[u] - sh.oy[a], bv.L[1]),
+                                 wrap1(zb[u] - sh.oz[a], bv.L[2]), sh.q64[a], pb[u]);
+          npairs++;
+        }
+        if (f[u] & 2)
+          e_new += pair_energy64(wrap1(xb[u] - sh.nx[a], bv.L[0]),
+                                 wrap1(yb[u] - sh.ny[a], bv.L[1]),
+                                 wrap1(zb[u] - sh.nz[a], bv.L[2]), sh.q64[a], pb[u]);
       }
-      if (f & 2)
-        e_new += pair_energy64(wrap1(xb - sh.nx[a], bv.L[0]),
-                               wrap1(yb - sh.ny[a], bv.L[1]),
-                               wrap1(zb - sh.nz[a], bv.L[2]), sh.q64[a], pb);
     }
   }
   __syncthreads();
@@ -270,11 +310,18 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
       }
     }
     int f = 0;
+    MolMeta mj{0, 0, 0, 0};
     if (j >= jlo && j != mol) {
       ncand++;
-      f = primary_in_range(bv, j, sh.ox, sh.oy, sh.oz, sh.pofs, sh.npri) ? 1 : 0;
+      mj = bv.meta[j];
+      int ps = j, pc = 1;  // one primary per molecule: no indirection
+      if (!bv.single_primary) {
+        ps = mj.pstart;
+        pc = bv.mol_pidx_count[j];
+      }
+      f = primary_in_range(bv, ps, pc, sh.ox, sh.oy, sh.oz, sh.pofs, sh.npri) ? 1 : 0;
       if (has_new)
-        f |= primary_in_range(bv, j, sh.nx, sh.ny, sh.nz, sh.pofs, sh.npri) ? 2 : 0;
+        f |= primary_in_range(bv, ps, pc, sh.nx, sh.ny, sh.nz, sh.pofs, sh.npri) ? 2 : 0;
     }
     int total;
     const int pos = block_scan_flag<NT>(f != 0, sh.scan, total);
@@ -283,8 +330,9 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
       nhits = 0;
     }
     if (f) {
-      sh.cand_j[nhits + pos] = j;
-      sh.cand_f[nhits + pos] = f;
+      sh.cand_start[nhits + pos] = mj.start;
+      sh.cand_param[nhits + pos] = mj.param;
+      sh.cand_lf[nhits + pos] = mj.len | (f << 8);
     }
     nhits += total;
     __syncthreads();
