@@ -55,9 +55,6 @@ int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y, double** z);
 /* .state files, full precision (%.17g) so a reload is bit exact. */
 int mcgpu_state_write(mcgpu_hostbox* h, const char* path, int64_t step);
 int mcgpu_state_read(const char* path, mcgpu_hostbox** out, int64_t* step);
-/* .config + z-matrix + OPLS parameter inputs (ConfigScanner.cpp,
- * ZmatrixScanner.cpp, OplsScanner.cpp, FileUtilities.cpp:285-642). */
-int mcgpu_config_read(const char* path, mcgpu_hostbox** out, int64_t* steps);
 
 /* The reference's LRC (SerialCalcs.cpp:37-108) and the K-species form. */
 int mcgpu_lrc_reference(const mcgpu_box_desc* d, double* e);
