@@ -25,6 +25,7 @@
 #include "eval.h"
 #include "fastmath.h"
 #include "mcgpu_host.h"
+#include "wave.h"
 
 namespace mcgpu {
 
@@ -59,171 +60,317 @@ struct BatchBox {
 
 // ------------------------------------------------------------- kernel
 
-// Proposed coordinates of the rigid move, built in LDS from the current
-// ones: every site except the pivot is rotated about the pivot by the X, Y,
-// Z Euler angles, then all sites are translated, then the molecule is
-// wrapped by its first primary site (SimulationStep.cpp:387-422, 545-565,
-// 710-729).  FMA contraction is off so the doubles match the host path.
+// LDS state of one work-group's box for a launch.
 template <int NT>
-__device__ void propose_device(EvalShared<NT>& sh, const BoxView& bv, int vertex,
-                               const double* d, const double* cs,
-                               const double* sn) {
+struct BatchShared {
+  EvalShared<NT> ev;
+  BatchBox tb;   // this box's descriptor, copied once per launch
+  BoxCtl c;      // its control block (wave 0, lane 0 updates it)
+  int32_t go;    // the current move runs (else the box stops this launch)
+  int32_t mol, start, pstart;
+  int32_t cells_live;  // cell lists in use and complete for this move
+  int32_t co_cell, cn_cell;  // first-primary cell before / after the move
+};
+
+// Wave 0: draws and builds the next trial move (Simulation.cpp:219-264 draw
+// order: molecule, vertex, dX, dY, dZ, rX, rY, rZ; SimulationStep.cpp:
+// 520-566 for the move itself), and prepares the neighbour source.
+//
+// The eight draws come from the wave-resident generator; lane k then forms
+// U_k = (b - a) * (r_k / RAND_MAX) + a for its own draw (MathLibrary.cpp:
+// 31-33) and lanes 5..7 take sin/cos of their angle, so the serial chain is
+// eight register draws long.  Lane k < len holds site k: the pivot and the
+// first primary are read across lanes, every site except the pivot is
+// rotated about the pivot by the X, Y, Z Euler angles, all are translated,
+// and the molecule is wrapped by its first primary (SimulationStep.cpp:
+// 387-422, 545-565, 710-729).  FMA contraction is off so the doubles match
+// the reference's.
+template <int NT>
+__device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s,
+                                           long long budget, long long t_start) {
 #pragma clang fp contract(off)
-  const int t = threadIdx.x;
-  const int len = sh.len;
-  const double px = sh.ox[vertex], py = sh.oy[vertex], pz = sh.oz[vertex];
-  for (int k = t; k < len; k += NT) {
-    double X = sh.ox[k], Y = sh.oy[k], Z = sh.oz[k];
-    if (k != vertex) {
-      X += -px;
-      Y += -py;
-      Z += -pz;
-      double u = Y, v = Z;  // rotateX
-      Y = u * cs[0] + v * sn[0];
-      Z = v * cs[0] - u * sn[0];
-      u = Z;
-      v = X;  // rotateY
-      Z = u * cs[1] + v * sn[1];
-      X = v * cs[1] - u * sn[1];
-      u = X;
-      v = Y;  // rotateZ
-      X = u * cs[2] + v * sn[2];
-      Y = v * cs[2] - u * sn[2];
-      X += px;
-      Y += py;
-      Z += pz;
+  const BoxView& bv = S.tb.bv;
+  EvalShared<NT>& sh = S.ev;
+  const int lane = wlane();
+  bool go = S.c.steps_done < S.c.next_stop;
+  if (go && budget > 0 && s > 0) go = (wall_clock64() - t_start) < budget;
+  if (!go) {
+    if (lane == 0) S.go = 0;
+    return;
+  }
+  const int n = bv.n_mol;
+  int R = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int r = wrng_next(g);
+    R = lane == k ? r : R;
+  }
+  const double q = (double)R / 2147483647;
+  double a = 0.0, b = 0.0;
+  if (lane == 0) {
+    b = (double)n;
+  } else if (lane >= 2 && lane < 5) {
+    a = -S.tb.max_t;
+    b = S.tb.max_t;
+  } else if (lane >= 5 && lane < 8) {
+    a = -S.tb.max_r;
+    b = S.tb.max_r;
+  }
+  const double U = (b - a) * q + a;
+  int mol = (int)readlane_f64(U, 0);
+  mol = mol >= n ? n - 1 : mol;  // rand() == RAND_MAX
+  // the molecule's record
+  int start, len, param, p0, pstart, npri;
+  int fault = 0;
+  if (bv.uni_len > 0) {
+    len = bv.uni_len;
+    start = mol * len;
+    param = bv.uni_param;
+    p0 = bv.uni_p0;
+    pstart = mol;
+    npri = 1;
+  } else {
+    const MolMeta m = bv.meta[mol];
+    start = m.start;
+    len = m.len;
+    param = m.param;
+    p0 = m.p0;
+    pstart = m.pstart;
+    npri = bv.single_primary ? 1 : bv.mol_pidx_count[mol];
+    if (len < 1 || len > kMaxSites) {
+      fault = MCGPU_FAULT_MOLLEN;
+      len = len < 1 ? 1 : kMaxSites;
     }
-    X += d[0];
-    Y += d[1];
-    Z += d[2];
-    sh.nx[k] = X;
-    sh.ny[k] = Y;
-    sh.nz[k] = Z;
+    npri = npri < 1 ? 1 : (npri > kMaxSites ? kMaxSites : npri);
   }
-  __syncthreads();
-  const int p0 = sh.pofs[0];
-  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const int cells_live = bv.mode == MCGPU_NB_CELL && bv.cells.overflow[0] == 0;
+  // committed stores of the previous move (other lanes' cell counts) are
+  // complete before this move's loads
+  __threadfence_block();
+  double xo = 0.0, yo = 0.0, zo = 0.0;
+  int pof = p0;
+  if (lane < len) {
+    xo = bv.x[start + lane];
+    yo = bv.y[start + lane];
+    zo = bv.z[start + lane];
+    sh.q64[lane] = bv.p64[param + lane];
+    sh.q32[lane] = bv.p32[param + lane];
+  }
+  if (npri > 1 && lane < npri) pof = bv.pidx[pstart + lane] - start;
+  // the rotation's draws while the sites load
+  const double rad = U * 3.14159265358979 / 180.0;
+  double sn, cs;
+  sincos_fp64(rad, sn, cs);
+  int vtx = (int)(((double)len - 0.0) * readlane_f64(q, 1) + 0.0);
+  vtx = vtx >= len ? len - 1 : vtx;
+  const double pvx = readlane_f64(xo, vtx), pvy = readlane_f64(yo, vtx),
+               pvz = readlane_f64(zo, vtx);
+  const double c0 = readlane_f64(cs, 5), s0 = readlane_f64(sn, 5);
+  const double c1 = readlane_f64(cs, 6), s1 = readlane_f64(sn, 6);
+  const double c2 = readlane_f64(cs, 7), s2 = readlane_f64(sn, 7);
+  double X = xo, Y = yo, Z = zo;
+  if (lane != vtx) {
+    X += -pvx;
+    Y += -pvy;
+    Z += -pvz;
+    double u = Y, v = Z;  // rotateX
+    Y = u * c0 + v * s0;
+    Z = v * c0 - u * s0;
+    u = Z;
+    v = X;  // rotateY
+    Z = u * c1 + v * s1;
+    X = v * c1 - u * s1;
+    u = X;
+    v = Y;  // rotateZ
+    X = u * c2 + v * s2;
+    Y = v * c2 - u * s2;
+    X += pvx;
+    Y += pvy;
+    Z += pvz;
+  }
+  X += readlane_f64(U, 2);
+  Y += readlane_f64(U, 3);
+  Z += readlane_f64(U, 4);
+  const int pf = readlane_i32(pof, 0);  // first primary
   {
-    const double qx = sh.nx[p0], qy = sh.ny[p0], qz = sh.nz[p0];
-    sx = qx < 0 ? bv.L[0] : (qx > bv.L[0] ? -bv.L[0] : 0.0);
-    sy = qy < 0 ? bv.L[1] : (qy > bv.L[1] ? -bv.L[1] : 0.0);
-    sz = qz < 0 ? bv.L[2] : (qz > bv.L[2] ? -bv.L[2] : 0.0);
+    const double qx = readlane_f64(X, pf), qy = readlane_f64(Y, pf),
+                 qz = readlane_f64(Z, pf);
+    const double sx = qx < 0 ? bv.L[0] : (qx > bv.L[0] ? -bv.L[0] : 0.0);
+    const double sy = qy < 0 ? bv.L[1] : (qy > bv.L[1] ? -bv.L[1] : 0.0);
+    const double sz = qz < 0 ? bv.L[2] : (qz > bv.L[2] ? -bv.L[2] : 0.0);
+    if (sx != 0.0) X += sx;
+    if (sy != 0.0) Y += sy;
+    if (sz != 0.0) Z += sz;
   }
-  __syncthreads();
-  for (int k = t; k < len; k += NT) {
-    if (sx != 0.0) sh.nx[k] += sx;
-    if (sy != 0.0) sh.ny[k] += sy;
-    if (sz != 0.0) sh.nz[k] += sz;
+  if (lane < len) {
+    sh.ox[lane] = xo;
+    sh.oy[lane] = yo;
+    sh.oz[lane] = zo;
+    sh.nx[lane] = X;
+    sh.ny[lane] = Y;
+    sh.nz[lane] = Z;
   }
-  __syncthreads();
+  if (lane < npri) sh.pofs[lane] = pof;
+  // neighbour source
+  int src = MCGPU_NB_BRUTE, nitems = n, ncells = 0, cold = 0, cnew = 0;
+  if (cells_live) {
+    const CellGrid& cg = bv.cells;
+    int co[3], cn[3];
+    co[0] = cell_coord(readlane_f64(xo, pf), cg.inv_w[0], cg.n[0]);
+    co[1] = cell_coord(readlane_f64(yo, pf), cg.inv_w[1], cg.n[1]);
+    co[2] = cell_coord(readlane_f64(zo, pf), cg.inv_w[2], cg.n[2]);
+    cn[0] = cell_coord(readlane_f64(X, pf), cg.inv_w[0], cg.n[0]);
+    cn[1] = cell_coord(readlane_f64(Y, pf), cg.inv_w[1], cg.n[1]);
+    cn[2] = cell_coord(readlane_f64(Z, pf), cg.inv_w[2], cg.n[2]);
+    cold = cell_index(cg, co[0], co[1], co[2]);
+    cnew = cell_index(cg, cn[0], cn[1], cn[2]);
+    int ni = 0;
+    const int nc = prepare_cells_wave(bv, sh, co, cn, &ni);
+    if (nc >= 0) {
+      src = MCGPU_NB_CELL;
+      nitems = ni;
+      ncells = nc;
+    }
+  }
+  if (nitems < 0 || nitems > n) {  // corrupt lists: evaluate nothing, stop
+    fault = MCGPU_FAULT_CANDIDATES;
+    nitems = 0;
+  }
+  if (lane == 0) {
+    sh.src = src;
+    sh.nitems = nitems;
+    sh.ncells = ncells;
+    sh.len = len;
+    sh.npri = npri;
+    sh.aw_log2 = lane_group_log2(len);
+    sh.fault = fault;
+    S.mol = mol;
+    S.start = start;
+    S.pstart = pstart;
+    S.cells_live = cells_live;
+    S.co_cell = cold;
+    S.cn_cell = cnew;
+    S.go = 1;
+  }
 }
 
-struct MoveDraw {
-  int32_t go;
-  int32_t mol;
-  int32_t vertex;
-  double d[3];
-  double cs[3];
-  double sn[3];
-};
+// Wave 0, after the evaluation: the Metropolis test (SimulationStep.cpp:
+// 794-803; the uniform is drawn only when dE >= 0), the per-box counters and
+// trace, and on acceptance the commit of the proposed sites, primary mirror
+// and cell lists.
+template <int NT>
+__device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
+  const BoxView& bv = S.tb.bv;
+  const EvalShared<NT>& sh = S.ev;
+  const int lane = wlane();
+  double eo, en;
+  unsigned long long wc;
+  unsigned np;
+  eval_totals<NT>(sh, eo, en, wc, np);
+  const double de = readlane_f64(en - eo, 0);
+  int acc;
+  if (de < 0) {
+    acc = 1;
+  } else {
+    acc = exp_fp64(-(de) / bv.kT) >= random_real(wrng_next(g), 0.0, 1.0);
+  }
+  const int mol = S.mol, start = S.start, len = sh.len;
+  if (lane == 0) {
+    const int64_t step = S.c.steps_done;
+    S.c.work_cand += (unsigned)(wc & 0xffffffffu);
+    S.c.work_sites += (unsigned)(wc >> 32);
+    S.c.work_self += 2 * len;
+    if (S.tb.record) {
+      S.tb.tr_mol[step] = mol;
+      S.tb.tr_de[step] = de;
+      S.tb.tr_acc[step] = (int8_t)acc;
+    }
+    if (acc) {
+      S.c.energy += de;
+      S.c.accepted++;
+    }
+    S.c.steps_done = step + 1;
+  }
+  if (!acc) return;
+  if (lane < len) {
+    bv.x[start + lane] = sh.nx[lane];
+    bv.y[start + lane] = sh.ny[lane];
+    bv.z[start + lane] = sh.nz[lane];
+  }
+  if (lane < sh.npri) {
+    const int o = sh.pofs[lane];
+    bv.px[S.pstart + lane] = sh.nx[o];
+    bv.py[S.pstart + lane] = sh.ny[o];
+    bv.pz[S.pstart + lane] = sh.nz[o];
+  }
+  // the cell of the first primary changes for a few per cent of moves; the
+  // old cell is the one its current position maps to (cells are assigned
+  // from the same positions), checked against the list before it is used
+  if (lane == 0 && S.cells_live && S.co_cell != S.cn_cell) {
+    const CellGrid& cg = bv.cells;
+    if (cg.mol_cell[mol] == S.co_cell)
+      cell_move(cg, mol, S.co_cell, S.cn_cell);
+    else
+      cg.overflow[0] = 1;  // lists disagree with positions: scan everything
+  }
+}
 
 // The persistent launch: work-group g owns box active[g] and runs up to
 // `sweeps` trial moves on it, stopping early at the box's step target, at a
 // pending NPT volume move, or when `budget` wall-clock ticks have passed
-// since the work-group started (0 = no time limit).
+// since the work-group started (0 = no time limit).  Wave 0 draws, builds,
+// decides and commits each move; all waves scan candidates and sum pairs.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
                                               BoxCtl* ctl,
                                               const int32_t* active,
                                               int sweeps, long long budget) {
-  __shared__ EvalShared<NT> sh;
-  __shared__ Rng31 rng;
-  __shared__ MoveDraw mv;
-  __shared__ BoxCtl c;
+  __shared__ BatchShared<NT> S;
+  constexpr int kWords = (int)(sizeof(BatchBox) / 4);
+  static_assert(sizeof(BatchBox) % 4 == 0 && kWords <= NT, "descriptor copy");
   const int t = threadIdx.x;
+  const bool w0 = t < 64;
   const int slot = __builtin_amdgcn_readfirstlane(active[blockIdx.x]);
-  const BatchBox& B = table[slot];
   const long long t_start = wall_clock64();
-  if (t == 0) {
-    rng = *B.rng;
-    c = ctl[slot];
-  }
+  if (t < kWords)
+    reinterpret_cast<int32_t*>(&S.tb)[t] = reinterpret_cast<const int32_t*>(table + slot)[t];
+  if (t == 0) S.c = ctl[slot];
   __syncthreads();
-  const BoxView& bv = B.bv;
-  for (int s = 0; s < sweeps; s++) {
-    if (t == 0) {
-#pragma clang fp contract(off)
-      bool go = c.steps_done < c.next_stop;
-      if (go && budget > 0 && s > 0) go = (wall_clock64() - t_start) < budget;
-      mv.go = go;
-      if (go) {
-        const int n = bv.n_mol;
-        int mol = (int)random_real(rng31_next(rng), 0, n);
-        mol = mol >= n ? n - 1 : mol;  // rand() == RAND_MAX
-        const int len = bv.mol_len[mol];
-        int vtx = (int)random_real(rng31_next(rng), 0, len);
-        vtx = vtx >= len ? len - 1 : vtx;
-        mv.mol = mol;
-        mv.vertex = vtx;
-        for (int k = 0; k < 3; k++)
-          mv.d[k] = random_real(rng31_next(rng), -B.max_t, B.max_t);
-        for (int k = 0; k < 3; k++) {
-          const double deg = random_real(rng31_next(rng), -B.max_r, B.max_r);
-          const double rad = deg * 3.14159265358979 / 180.0;
-          sincos_fp64(rad, mv.sn[k], mv.cs[k]);
-        }
-      }
-    }
-    __syncthreads();
-    if (!mv.go) break;
-    const int mol = mv.mol;
-    load_molecule<NT>(bv, mol, sh);
-    __syncthreads();
-    propose_device<NT>(sh, bv, mv.vertex, mv.d, mv.cs, mv.sn);
-    double eo, en;
-    EvalWork w;
-    if (B.precision == MCGPU_MIXED)
-      eval_molecule<NT, true>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
-    else
-      eval_molecule<NT, false>(bv, mol, 0, true, sh, eo, en, nullptr, &w);
-    if (sh.fault) {  // block-uniform (LDS): stop this box, report it
-      if (t == 0) c.fault = sh.fault;
-      break;
-    }
-    if (t == 0) {
-      c.work_cand += w.cand;
-      c.work_sites += w.sites;
-      c.work_self += 2 * sh.len;
-      const double de = en - eo;
-      int acc;
-      if (de < 0) {
-        acc = 1;
-      } else {
-        acc = exp_fp64(-(de) / bv.kT) >= random_real(rng31_next(rng), 0.0, 1.0);
-      }
-      if (B.record) {
-        B.tr_mol[c.steps_done] = mol;
-        B.tr_de[c.steps_done] = de;
-        B.tr_acc[c.steps_done] = (int8_t)acc;
-      }
-      if (acc) {
-        c.energy += en - eo;
-        c.accepted++;
-      }
-      c.steps_done++;
-      sh.flag = acc;
-    }
-    __syncthreads();
-    if (sh.flag) commit_molecule<NT>(bv, mol, sh);
-    __syncthreads();
+  WaveRng g{0, 0, 0};
+  if (w0) {
+    const Rng31* rs = S.tb.rng;
+    const int lane = wlane();
+    g.s = lane < 31 ? rs->s[lane] : 0;
+    g.f = __builtin_amdgcn_readfirstlane(rs->f);
+    g.r = __builtin_amdgcn_readfirstlane(rs->r);
   }
-  if (t == 0) {
-    *B.rng = rng;
-    if (c.steps_done >= c.steps_target || c.fault)
-      c.status = kDone;
-    else if (c.steps_done >= c.next_stop)
-      c.status = kVolumePending;
-    ctl[slot] = c;
+  const BoxView& bv = S.tb.bv;
+  const bool mixed = S.tb.precision == MCGPU_MIXED;
+  for (int s = 0; s < sweeps; s++) {
+    if (w0) front_move<NT>(S, g, s, budget, t_start);
+    __syncthreads();
+    if (!S.go || S.ev.fault) break;
+    const int mol = __builtin_amdgcn_readfirstlane(S.mol);
+    if (mixed)
+      eval_core<NT, true>(bv, S.ev, mol, 0, true);
+    else
+      eval_core<NT, false>(bv, S.ev, mol, 0, true);
+    if (w0) finish_move<NT>(S, g);
+  }
+  if (w0) {
+    const int lane = wlane();
+    Rng31* rs = S.tb.rng;
+    if (lane < 31) rs->s[lane] = g.s;
+    if (lane == 0) {
+      rs->f = g.f;
+      rs->r = g.r;
+      if (S.ev.fault && S.go) S.c.fault = S.ev.fault;
+      if (S.c.steps_done >= S.c.steps_target || S.c.fault)
+        S.c.status = kDone;
+      else if (S.c.steps_done >= S.c.next_stop)
+        S.c.status = kVolumePending;
+      ctl[slot] = S.c;
+    }
   }
 }
 
@@ -791,6 +938,9 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   v.n_mol = d->n_molecules;
   v.n_site = d->n_sites;
   v.max_len = pt.max_len;
+  v.uni_len = pt.uni_len;
+  v.uni_param = pt.uni_param;
+  v.uni_p0 = pt.uni_p0;
   v.mode = MCGPU_NB_BRUTE;
   for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
   v.cutoff2 = d->cutoff * d->cutoff;

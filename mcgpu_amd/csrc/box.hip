@@ -330,8 +330,17 @@ ParamTable build_param_table(const mcgpu_box_desc& d) {
       it = sig.emplace(key, base).first;
     }
     t.mol_param[m] = it->second;
-    t.meta[m] = MolMeta{s0, len, it->second, d.mol_pidx_start[m]};
+    const int32_t pf = d.primary_index[d.mol_pidx_start[m]] - s0;
+    t.meta[m] = MolMeta{s0, (int16_t)len, (int16_t)pf, it->second, d.mol_pidx_start[m]};
     if (d.mol_pidx_count[m] != 1 || d.mol_pidx_start[m] != m) t.single_primary = 0;
+    if (m == 0) {
+      t.uni_len = len;
+      t.uni_param = it->second;
+      t.uni_p0 = pf;
+    }
+    if (s0 != m * t.uni_len || len != t.uni_len || it->second != t.uni_param ||
+        pf != t.uni_p0)
+      t.uniform = false;
     // primary spread (rigid molecules: fixed)
     const int32_t p0 = d.primary_index[d.mol_pidx_start[m]];
     for (int32_t k = 1; k < d.mol_pidx_count[m]; k++) {
@@ -340,6 +349,8 @@ ParamTable build_param_table(const mcgpu_box_desc& d) {
       t.spread = std::max(t.spread, std::sqrt(dx * dx + dy * dy + dz * dz));
     }
   }
+  if (!t.single_primary) t.uniform = false;
+  if (!t.uniform) t.uni_len = t.uni_param = t.uni_p0 = 0;
   return t;
 }
 
@@ -592,6 +603,9 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   v.n_mol = d->n_molecules;
   v.n_site = d->n_sites;
   v.max_len = pt.max_len;
+  v.uni_len = pt.uni_len;
+  v.uni_param = pt.uni_param;
+  v.uni_p0 = pt.uni_p0;
   v.mode = MCGPU_NB_BRUTE;
   for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
   v.cutoff2 = d->cutoff * d->cutoff;

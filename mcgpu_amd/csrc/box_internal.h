@@ -43,6 +43,9 @@ struct ParamTable {
   int32_t single_primary = 1;
   double spread = 0.0;  // max distance of a primary from the first primary
   int32_t max_len = 0;
+  // every molecule alike (see BoxView::uni_len); zeros otherwise
+  bool uniform = true;
+  int32_t uni_len = 0, uni_param = 0, uni_p0 = 0;
 };
 ParamTable build_param_table(const mcgpu_box_desc& d);
 

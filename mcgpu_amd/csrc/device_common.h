@@ -31,11 +31,15 @@ struct SiteParam32 {
   float ss, se, q, real;
 };
 
-// Per-molecule record read with one 16-byte load in the candidate scan:
-// first site, site count, first entry in the site-parameter table, first
-// entry in the primary list.
+// Per-molecule record read with one 16-byte load: first site, site count,
+// offset of the first primary site within the molecule, first entry in the
+// site-parameter table, first entry in the primary list.
 struct alignas(16) MolMeta {
-  int32_t start, len, param, pstart;
+  int32_t start;
+  int16_t len;
+  int16_t p0;
+  int32_t param;
+  int32_t pstart;
 };
 
 struct CellGrid {
@@ -88,7 +92,12 @@ struct BoxView {
   int32_t max_len;
   int32_t mode;  // MCGPU_NB_*
   int32_t single_primary;  // every molecule j has one primary, entry j
-  int32_t pad0;
+  // Uniform boxes (every molecule alike: uni_len sites starting at
+  // m * uni_len, parameters from entry uni_param, one primary at offset
+  // uni_p0): a molecule's record follows from its index without a load.
+  int32_t uni_len;  // 0 when the box is not uniform
+  int32_t uni_param;
+  int32_t uni_p0;
   double L[3];
   double cutoff2;
   double kT;

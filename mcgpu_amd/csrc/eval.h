@@ -1,28 +1,32 @@
 // Work-group level evaluation of one molecule's intermolecular energy at its
 // current and (optionally) proposed position, in one pass over candidates.
 //
-// Phase 1 (candidate scan): lanes take candidate molecules j from the
-// neighbour source -- every molecule, the occupied slots of the cells around
-// the old and proposed cells (enumerated through a prefix over the cell
-// counts, so no lane looks at an empty slot), or the Verlet list -- test the
+// Preparation (wave 0): the neighbour source -- every molecule, the occupied
+// slots of the cells spanning the old and proposed cell (their occupancies
+// prefix-summed by a DPP wave scan, with an item -> cell map in LDS), or the
+// Verlet list.
+// Candidate scan (all lanes): each lane takes one candidate j, tests the
 // primary-site cutoff at the old and proposed position (SimulationStep.cpp:
-// 317-330), and the hits are compacted into LDS in candidate order by a
-// ballot/popcount block scan.
-// Phase 2 (pair sum): lanes are grouped Aw = next_pow2(m_i) at a time; a
-// group takes one (hit, site b) item and each lane of it one site a of the
-// moving molecule, so every (a, b) pair of every hit is one lane-step
-// (BruteForceStep.cpp:48-79).  Sums are FP64; one fixed-order block
-// reduction ends the evaluation, so results are deterministic for a given
-// work-group size.
+// 317-330), and the hits are compacted into LDS in candidate order (ballot +
+// mbcnt, one barrier per pass).
+// Pair phase: lanes are grouped Aw = next_pow2(m_i) at a time; a group takes
+// one (hit, site b) item and each lane of it one site a of the moving
+// molecule, so every (a, b) pair of every hit is one lane-step
+// (BruteForceStep.cpp:48-79).  Sums are FP64; per-wave DPP sums and a
+// fixed-order sum over waves end the evaluation, so results are
+// deterministic for a given work-group size.
 #pragma once
 
 #include "device_common.h"
+#include "wave.h"
 
 namespace mcgpu {
 
 template <int NT>
 struct EvalShared {
   static constexpr int kCap = NT * 2 < 256 ? 256 : NT * 2;
+  static constexpr int kItemCap = 2048;  // items with a cell entry in item_cell
+  static constexpr int kNW = NT / 64;
   double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
   double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
   SiteParam64 q64[kMaxSites];
@@ -32,19 +36,27 @@ struct EvalShared {
   int32_t cand_start[kCap];  // first site
   int32_t cand_param[kCap];  // first site-parameter entry
   int32_t cand_lf[kCap];     // site count | flags << 8 (bit0 old, bit1 new)
-  int32_t cell_id[64];   // cells of the current range (at most 4x4x4)
-  int32_t cell_pre[65];  // prefix of their occupancies
-  double red[NT / 64];
-  double red2[NT / 64];
-  unsigned long long redc[NT / 64];
-  int32_t scan[NT / 64];
+  int32_t cell_id[64];       // cells of the current range (at most 4x4x4)
+  int32_t cell_pre[65];      // prefix of their occupancies
+  uint8_t item_cell[kItemCap];  // range cell of each candidate item
+  double red[kNW];
+  double red2[kNW];
+  unsigned long long redc[kNW];  // sites << 32 | candidates, per wave
+  unsigned redp[kNW];            // site pairs at the old position, per wave
+  int32_t scan[2][kNW];          // hit counts per wave, double-buffered
   int32_t npri;
   int32_t len;
-  int32_t nreal;
   int32_t aw_log2;  // log2 of the lane-group width of the pair phase
   int32_t flag;
-  int32_t fault;    // MCGPU_FAULT_* raised by the last evaluation (0 = none)
+  int32_t fault;    // MCGPU_FAULT_* raised by the last preparation (0 = none)
+  int32_t src;      // neighbour source of the current evaluation
+  int32_t nitems;   // candidate items it holds
+  int32_t ncells;   // cells of the range (cell source)
 };
+
+__device__ __forceinline__ int lane_group_log2(int len) {
+  return len > 1 ? 32 - __clz(len - 1) : 0;
+}
 
 // Loads molecule `mol`'s current sites (and parameters) into LDS.
 template <int NT>
@@ -70,12 +82,7 @@ __device__ __forceinline__ void load_molecule(const BoxView& bv, int mol,
     for (int k = 0; k < pc; k++) sh.pofs[k] = bv.pidx[ps + k] - start;
     sh.npri = pc;
     sh.len = len;
-    int nr = 0;
-    for (int k = 0; k < len; k++) nr += bv.p64[pbase + k].real != 0.0;
-    sh.nreal = nr;
-    int lg = 0;
-    while ((1 << lg) < len) lg++;
-    sh.aw_log2 = lg;
+    sh.aw_log2 = lane_group_log2(len);
   }
 }
 
@@ -103,12 +110,11 @@ __device__ __forceinline__ bool primary_in_range(const BoxView& bv, int ps, int 
 
 // Sums the pair energies of the compacted hits into e_old / e_new.  Each
 // lane handles two (hit, site b) items per iteration so the site loads of
-// both are in flight together.
+// both are in flight together.  No barrier: callers order LDS reuse.
 template <int NT, bool MIXED>
 __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
                                            int nhits, double& e_old,
-                                           double& e_new,
-                                           unsigned long long& npairs,
+                                           double& e_new, unsigned& npairs,
                                            unsigned& nsites) {
   const int M = bv.max_len;
   const int len = sh.len;
@@ -192,7 +198,6 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
       }
     }
   }
-  __syncthreads();
 }
 
 // Neighbour source actually usable for this evaluation (block-uniform).
@@ -214,6 +219,49 @@ __device__ __forceinline__ int pick_source(const BoxView& bv, const double* nx,
   return MCGPU_NB_BRUTE;
 }
 
+// Wave 0 (all 64 lanes): the cells around old cell co and proposed cell cn,
+// their occupancies, the prefix and the item -> cell map.  Returns the cell
+// count, or -1 when the range exceeds 64 cells (the caller then scans every
+// molecule); *nitems receives the candidate count.
+template <int NT>
+__device__ __forceinline__ int prepare_cells_wave(const BoxView& bv,
+                                                  EvalShared<NT>& sh,
+                                                  const int co[3], const int cn[3],
+                                                  int* nitems) {
+  const CellGrid& g = bv.cells;
+  const CellRange range = make_range(g, co, cn);
+  const int ncells = range.total();
+  if (ncells > 64) return -1;
+  const int lane = wlane();
+  int n = 0, c = 0;
+  if (lane < ncells) {
+    c = range.cell(g, lane);
+    n = g.count[c];
+    n = n < g.cap ? n : g.cap;
+    sh.cell_id[lane] = c;
+  }
+  const int incl = wave_incl_scan_i32(n);
+  sh.cell_pre[lane + 1] = incl;
+  if (lane == 0) sh.cell_pre[0] = 0;
+  const int end = incl < EvalShared<NT>::kItemCap ? incl : EvalShared<NT>::kItemCap;
+  for (int k = incl - n; k < end; k++) sh.item_cell[k] = (uint8_t)lane;
+  *nitems = readlane_i32(incl, 63);
+  return ncells;
+}
+
+// The range cell holding candidate `item` of the cell source.
+template <int NT>
+__device__ __forceinline__ int item_cell_of(const EvalShared<NT>& sh, int item,
+                                            int ncells) {
+  if (item < EvalShared<NT>::kItemCap) return sh.item_cell[item];
+  int lo = 0, hi = ncells;  // last cell whose prefix <= item
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sh.cell_pre[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // Work counters of one evaluation (block-uniform): candidate molecules
 // examined and sites of in-range molecules loaded; they give the kernel's
 // algorithmic bytes (24 B per candidate primary, 24 B per site).
@@ -222,86 +270,28 @@ struct EvalWork {
   unsigned sites;
 };
 
-// Evaluates E_mol over candidates j >= jlo (j != mol) at the LDS-held old
-// position, and at the proposed position when has_new.  Must be called by
-// all NT threads after load_molecule (+ proposed coords) and a barrier.
-// Results are block-uniform.
+// Candidate scan + pair phase over the prepared source (sh.src / nitems /
+// ncells and the cell tables), for molecule `mol` at the LDS-held old
+// position and, when has_new, the proposed one.  Called by all NT threads
+// after a barrier that follows the preparation.  Leaves per-wave partial
+// sums in sh.red / red2 / redc / redp and ends with a barrier.
 template <int NT, bool MIXED>
-__device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
-                              bool has_new, EvalShared<NT>& sh, double& e_old,
-                              double& e_new, unsigned long long* pair_count,
-                              EvalWork* work = nullptr) {
+__device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
+                                          int mol, int jlo, bool has_new) {
   const int t = threadIdx.x;
-  const int p0 = sh.pofs[0];
-  int src = pick_source(bv, sh.nx, sh.ny, sh.nz, p0, mol, has_new);
-  int nitems, ncells = 0;
-  CellRange range{};
-  if (src == MCGPU_NB_CELL) {
-    const CellGrid& g = bv.cells;
-    int co[3], cn[3];
-    co[0] = cell_coord(sh.ox[p0], g.inv_w[0], g.n[0]);
-    co[1] = cell_coord(sh.oy[p0], g.inv_w[1], g.n[1]);
-    co[2] = cell_coord(sh.oz[p0], g.inv_w[2], g.n[2]);
-    if (has_new) {
-      cn[0] = cell_coord(sh.nx[p0], g.inv_w[0], g.n[0]);
-      cn[1] = cell_coord(sh.ny[p0], g.inv_w[1], g.n[1]);
-      cn[2] = cell_coord(sh.nz[p0], g.inv_w[2], g.n[2]);
-    } else {
-      cn[0] = co[0];
-      cn[1] = co[1];
-      cn[2] = co[2];
-    }
-    range = make_range(g, co, cn);
-    ncells = range.total();
-    if (ncells > 64) src = MCGPU_NB_BRUTE;  // see make_range (uniform)
-  }
-  if (src == MCGPU_NB_CELL) {
-    const CellGrid& g = bv.cells;
-    // ncells <= 64: one wave builds the prefix of the cell occupancies
-    if (t < 64) {
-      int n = 0;
-      if (t < ncells) {
-        const int c = range.cell(g, t);
-        n = g.count[c];
-        n = n < g.cap ? n : g.cap;
-        sh.cell_id[t] = c;
-      }
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(n, o);
-        if (t >= o) n += u;
-      }
-      sh.cell_pre[t + 1] = n;
-      if (t == 0) sh.cell_pre[0] = 0;
-    }
-    __syncthreads();
-    nitems = sh.cell_pre[ncells];
-  } else if (src == MCGPU_NB_VERLET) {
-    nitems = bv.verlet.count[mol];
-  } else {
-    nitems = bv.n_mol - jlo;
-  }
-  // A neighbour source never holds more candidates than molecules; if it
-  // claims so its lists are corrupt: record it and evaluate nothing.
-  if (nitems < 0 || nitems > bv.n_mol) {
-    if (t == 0) sh.fault = MCGPU_FAULT_CANDIDATES;
-    nitems = 0;
-  }
-
+  const int src = __builtin_amdgcn_readfirstlane(sh.src);
+  const int nitems = __builtin_amdgcn_readfirstlane(sh.nitems);
+  const int ncells = __builtin_amdgcn_readfirstlane(sh.ncells);
+  const int npri = __builtin_amdgcn_readfirstlane(sh.npri);
   double eo = 0.0, en = 0.0;
-  unsigned long long np = 0;
-  unsigned ncand = 0, nsites = 0;
-  int nhits = 0;
+  unsigned ncand = 0, nsites = 0, npairs = 0;
+  int nhits = 0, par = 0;
   for (int base = 0; base < nitems; base += NT) {
     const int item = base + t;
     int j = -1;
     if (item < nitems) {
       if (src == MCGPU_NB_CELL) {
-        int lo = 0, hi = ncells;  // last cell whose prefix <= item
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (sh.cell_pre[mid] <= item) lo = mid; else hi = mid;
-        }
+        const int lo = item_cell_of(sh, item, ncells);
         j = bv.cells.slots[sh.cell_id[lo] * bv.cells.cap + (item - sh.cell_pre[lo])];
       } else if (src == MCGPU_NB_VERLET) {
         j = bv.verlet.list[mol * bv.verlet.cap + item];
@@ -310,7 +300,7 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
       }
     }
     int f = 0;
-    MolMeta mj{0, 0, 0, 0};
+    MolMeta mj{0, 0, 0, 0, 0};
     if (j >= jlo && j != mol) {
       ncand++;
       mj = bv.meta[j];
@@ -319,52 +309,155 @@ __device__ void eval_molecule(const BoxView& bv, int mol, int jlo,
         ps = mj.pstart;
         pc = bv.mol_pidx_count[j];
       }
-      f = primary_in_range(bv, ps, pc, sh.ox, sh.oy, sh.oz, sh.pofs, sh.npri) ? 1 : 0;
+      f = primary_in_range(bv, ps, pc, sh.ox, sh.oy, sh.oz, sh.pofs, npri) ? 1 : 0;
       if (has_new)
-        f |= primary_in_range(bv, ps, pc, sh.nx, sh.ny, sh.nz, sh.pofs, sh.npri) ? 2 : 0;
+        f |= primary_in_range(bv, ps, pc, sh.nx, sh.ny, sh.nz, sh.pofs, npri) ? 2 : 0;
     }
-    int total;
-    const int pos = block_scan_flag<NT>(f != 0, sh.scan, total);
-    if (nhits + total > EvalShared<NT>::kCap) {
-      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np, nsites);
+    // compaction: rank within the wave, wave counts through LDS
+    const unsigned long long bal = __ballot(f != 0);
+    if (wlane() == 0) sh.scan[par][wave_id()] = __popcll(bal);
+    __syncthreads();
+    int pre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+      const int c = sh.scan[par][w];
+      pre += (w < wave_id()) ? c : 0;
+      total += c;
+    }
+    par ^= 1;
+    if (nhits + total > EvalShared<NT>::kCap) {  // block-uniform
+      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs, nsites);
+      __syncthreads();  // every lane done with the hits before they are reused
       nhits = 0;
     }
     if (f) {
-      sh.cand_start[nhits + pos] = mj.start;
-      sh.cand_param[nhits + pos] = mj.param;
-      sh.cand_lf[nhits + pos] = mj.len | (f << 8);
+      const int pos = nhits + pre + wave_rank(bal);
+      sh.cand_start[pos] = mj.start;
+      sh.cand_param[pos] = mj.param;
+      sh.cand_lf[pos] = mj.len | (f << 8);
     }
     nhits += total;
-    __syncthreads();
   }
-  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, np, nsites);
-  // one fixed-order reduction of both energies and the work counters
-  unsigned long long wc = ((unsigned long long)nsites << 32) | ncand;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    eo += __shfl_xor(eo, o);
-    en += __shfl_xor(en, o);
-    wc += __shfl_xor(wc, o);
-  }
-  if (lane_id() == 0) {
+  __syncthreads();
+  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs, nsites);
+  eo = wave_sum_f64(eo);
+  en = wave_sum_f64(en);
+  const unsigned wcand = wave_sum_u32(ncand);
+  const unsigned wsites = wave_sum_u32(nsites);
+  const unsigned wpairs = wave_sum_u32(npairs);
+  if (wlane() == 0) {
     sh.red[wave_id()] = eo;
     sh.red2[wave_id()] = en;
-    sh.redc[wave_id()] = wc;
+    sh.redc[wave_id()] = ((unsigned long long)wsites << 32) | wcand;
+    sh.redp[wave_id()] = wpairs;
   }
   __syncthreads();
-  const double so = wave_sum_partials<NT / 64>(sh.red);
-  const double sn = wave_sum_partials<NT / 64>(sh.red2);
-  const unsigned long long sc = wave_sum_partials<NT / 64>(sh.redc);
-  __syncthreads();
+}
+
+// Fixed-order totals of the per-wave partials left by eval_core.
+template <int NT>
+__device__ __forceinline__ void eval_totals(const EvalShared<NT>& sh, double& e_old,
+                                            double& e_new, unsigned long long& counts,
+                                            unsigned& pairs) {
+  double so = 0.0, sn = 0.0;
+  unsigned long long c = 0;
+  unsigned p = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    so += sh.red[w];
+    sn += sh.red2[w];
+    c += sh.redc[w];
+    p += sh.redp[w];
+  }
   e_old = so;
-  e_new = has_new ? sn : 0.0;
-  if (work) {
-    work->cand = (unsigned)(sc & 0xffffffffu);
-    work->sites = (unsigned)(sc >> 32);
+  e_new = sn;
+  counts = c;
+  pairs = p;
+}
+
+// Evaluates E_mol over candidates j >= jlo (j != mol) at the LDS-held old
+// position, and at the proposed position when has_new.  Must be called by
+// all NT threads after load_molecule (+ proposed coords) and a barrier.
+// Results are block-uniform.
+template <int NT, bool MIXED>
+__device__ __forceinline__ void eval_molecule(const BoxView& bv, int mol, int jlo,
+                                              bool has_new, EvalShared<NT>& sh,
+                                              double& e_old, double& e_new,
+                                              unsigned long long* pair_count,
+                                              EvalWork* work = nullptr) {
+  if (threadIdx.x < 64) {  // wave 0 prepares the neighbour source
+    const int p0 = sh.pofs[0];
+    int src = pick_source(bv, sh.nx, sh.ny, sh.nz, p0, mol, has_new);
+    int nitems = 0, ncells = 0;
+    if (src == MCGPU_NB_CELL) {
+      const CellGrid& g = bv.cells;
+      int co[3], cn[3];
+      co[0] = cell_coord(sh.ox[p0], g.inv_w[0], g.n[0]);
+      co[1] = cell_coord(sh.oy[p0], g.inv_w[1], g.n[1]);
+      co[2] = cell_coord(sh.oz[p0], g.inv_w[2], g.n[2]);
+      if (has_new) {
+        cn[0] = cell_coord(sh.nx[p0], g.inv_w[0], g.n[0]);
+        cn[1] = cell_coord(sh.ny[p0], g.inv_w[1], g.n[1]);
+        cn[2] = cell_coord(sh.nz[p0], g.inv_w[2], g.n[2]);
+      } else {
+        cn[0] = co[0];
+        cn[1] = co[1];
+        cn[2] = co[2];
+      }
+      ncells = prepare_cells_wave(bv, sh, co, cn, &nitems);
+      if (ncells < 0) {
+        src = MCGPU_NB_BRUTE;
+        ncells = 0;
+      }
+    }
+    if (src == MCGPU_NB_VERLET) nitems = bv.verlet.count[mol];
+    if (src == MCGPU_NB_BRUTE) nitems = bv.n_mol - jlo;
+    // A neighbour source never holds more candidates than molecules; if it
+    // claims so its lists are corrupt: record it and evaluate nothing.
+    int fault = 0;
+    if (nitems < 0 || nitems > bv.n_mol) {
+      fault = MCGPU_FAULT_CANDIDATES;
+      nitems = 0;
+    }
+    if (threadIdx.x == 0) {
+      sh.src = src;
+      sh.nitems = nitems;
+      sh.ncells = ncells;
+      if (fault) sh.fault = fault;
+    }
   }
-  if (pair_count) {
-    const double c = block_sum<NT>((double)np, sh.red);
-    if (t == 0) *pair_count = (unsigned long long)c;
+  __syncthreads();
+  eval_core<NT, MIXED>(bv, sh, mol, jlo, has_new);
+  unsigned long long c;
+  unsigned p;
+  eval_totals<NT>(sh, e_old, e_new, c, p);
+  if (!has_new) e_new = 0.0;
+  if (work) {
+    work->cand = (unsigned)(c & 0xffffffffu);
+    work->sites = (unsigned)(c >> 32);
+  }
+  if (pair_count) *pair_count = p;
+}
+
+// Moves molecule `mol` from cell cold to cell cnew in the cell lists (one
+// lane): swap-remove from the old cell, append to the new one; a full cell
+// marks the lists incomplete (scans then fall back to every molecule).
+__device__ __forceinline__ void cell_move(const CellGrid& g, int mol, int cold,
+                                          int cnew) {
+  const int slot = g.mol_slot[mol];
+  const int last = g.count[cold] - 1;
+  const int lm = g.slots[cold * g.cap + last];
+  g.slots[cold * g.cap + slot] = lm;
+  g.mol_slot[lm] = slot;
+  g.count[cold] = last;
+  const int n = g.count[cnew];
+  if (n < g.cap) {
+    g.slots[cnew * g.cap + n] = mol;
+    g.mol_slot[mol] = n;
+    g.count[cnew] = n + 1;
+    g.mol_cell[mol] = cnew;
+  } else {
+    g.overflow[0] = 1;
   }
 }
 
@@ -394,23 +487,7 @@ __device__ void commit_molecule(const BoxView& bv, int mol,
                                   cell_coord(sh.ny[p0], g.inv_w[1], g.n[1]),
                                   cell_coord(sh.nz[p0], g.inv_w[2], g.n[2]));
       const int cold = g.mol_cell[mol];
-      if (cnew != cold) {
-        const int slot = g.mol_slot[mol];
-        const int last = g.count[cold] - 1;
-        const int lm = g.slots[cold * g.cap + last];
-        g.slots[cold * g.cap + slot] = lm;
-        g.mol_slot[lm] = slot;
-        g.count[cold] = last;
-        const int n = g.count[cnew];
-        if (n < g.cap) {
-          g.slots[cnew * g.cap + n] = mol;
-          g.mol_slot[mol] = n;
-          g.count[cnew] = n + 1;
-          g.mol_cell[mol] = cnew;
-        } else {
-          g.overflow[0] = 1;  // lists incomplete from here: scans fall back
-        }
-      }
+      if (cnew != cold) cell_move(g, mol, cold, cnew);
     }
     if (bv.mode == MCGPU_NB_VERLET) {
       const double* r = bv.verlet.ref + 3 * mol;
