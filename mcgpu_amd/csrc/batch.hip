@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "box_internal.h"
+#include "cells.h"
 #include "eval.h"
 #include "fastmath.h"
 #include "mcgpu_host.h"
@@ -52,10 +53,22 @@ struct BatchBox {
   int32_t* tr_mol;
   double* tr_de;
   int8_t* tr_acc;
+  double* partial;  // [n_mol] per-molecule energies of a full-energy pass
+  double* total;    // [4] full-energy totals
+  double* bx;       // [n_site] x3: coordinates before an NPT rescale
+  double* by;
+  double* bz;
   double max_t;
   double max_r;
   int32_t precision;
   int32_t record;
+};
+
+// One box of a batched full-energy / rescale / cell pass.
+struct VolItem {
+  int32_t slot;
+  int32_t pad;
+  double f;  // edge scale factor of the rescale
 };
 
 // ------------------------------------------------------------- kernel
@@ -315,7 +328,7 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
   }
 }
 
-// The persistent launch: work-group g owns box active[g] and runs up to
+// The persistent launch: work-group g owns box active[1 + g] and runs up to
 // `sweeps` trial moves on it, stopping early at the box's step target, at a
 // pending NPT volume move, or when `budget` wall-clock ticks have passed
 // since the work-group started (0 = no time limit).  Wave 0 draws, builds,
@@ -330,7 +343,9 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   static_assert(sizeof(BatchBox) % 4 == 0 && kWords <= NT, "descriptor copy");
   const int t = threadIdx.x;
   const bool w0 = t < 64;
-  const int slot = __builtin_amdgcn_readfirstlane(active[blockIdx.x]);
+  // active = [count, slot 0, slot 1, ...]; the grid is rounded up
+  if ((int)blockIdx.x >= __builtin_amdgcn_readfirstlane(active[0])) return;
+  const int slot = __builtin_amdgcn_readfirstlane(active[1 + blockIdx.x]);
   const long long t_start = wall_clock64();
   if (t < kWords)
     reinterpret_cast<int32_t*>(&S.tb)[t] = reinterpret_cast<const int32_t*>(table + slot)[t];
@@ -393,6 +408,132 @@ __global__ void k_restore_coords(double* x, double* y, double* z,
     y[i] = by[i];
     z[i] = bz[i];
   }
+}
+
+// ---------------------------------------- batched NPT passes (many boxes)
+//
+// One launch covers every box of a list (blockIdx.y = list entry), so the
+// volume moves of all boxes due in a tick cost a handful of launches.
+
+// Full-system energy (SimulationStep.cpp:40-71): work-group x of entry y sums
+// E_mol(mol) over partners j > mol for molecules x, x + gridDim.x, ...
+template <int NT>
+__global__ __launch_bounds__(NT) void k_energy_multi(const BatchBox* table,
+                                                     const VolItem* items) {
+  __shared__ EvalShared<NT> sh;
+  const BatchBox& B = table[items[blockIdx.y].slot];
+  const BoxView& bv = B.bv;
+  const bool mixed = B.precision == MCGPU_MIXED;
+  for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
+    load_molecule<NT>(bv, mol, sh);
+    __syncthreads();
+    double eo, en;
+    if (mixed)
+      eval_molecule<NT, true>(bv, mol, mol + 1, false, sh, eo, en, nullptr);
+    else
+      eval_molecule<NT, false>(bv, mol, mol + 1, false, sh, eo, en, nullptr);
+    if (threadIdx.x == 0) B.partial[mol] = eo;
+    __syncthreads();
+  }
+}
+
+// Fixed-order sum of entry blockIdx.x's partials into total[which] and
+// out[2 * entry + which].
+__global__ __launch_bounds__(256) void k_reduce_multi(const BatchBox* table,
+                                                      const VolItem* items, int which,
+                                                      double* out) {
+  __shared__ double red[4];
+  const BatchBox& B = table[items[blockIdx.x].slot];
+  const int n = B.bv.n_mol;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += B.partial[i];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) {
+    B.total[which] = s;
+    out[2 * blockIdx.x + which] = s;
+  }
+}
+
+// NPT rescale of entry y: molecule m is translated so its first primary
+// moves from p to f p (internal geometry untouched); its sites are backed up
+// first and its primary mirror entries refreshed.
+__global__ __launch_bounds__(256) void k_scale_multi(const BatchBox* table,
+                                                     const VolItem* items) {
+  const VolItem it = items[blockIdx.y];
+  const BatchBox& B = table[it.slot];
+  const BoxView& bv = B.bv;
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= bv.n_mol) return;
+  const int p = bv.pidx[bv.mol_pidx_start[m]];
+  const double f = it.f;
+  const double sx = bv.x[p] * f - bv.x[p];
+  const double sy = bv.y[p] * f - bv.y[p];
+  const double sz = bv.z[p] * f - bv.z[p];
+  const int s0 = bv.mol_start[m], s1 = s0 + bv.mol_len[m];
+  for (int s = s0; s < s1; s++) {
+    B.bx[s] = bv.x[s];
+    B.by[s] = bv.y[s];
+    B.bz[s] = bv.z[s];
+    bv.x[s] += sx;
+    bv.y[s] += sy;
+    bv.z[s] += sz;
+  }
+  const int ps = bv.mol_pidx_start[m], pe = ps + bv.mol_pidx_count[m];
+  for (int k = ps; k < pe; k++) {
+    const int q = bv.pidx[k];
+    bv.px[k] = bv.x[q];
+    bv.py[k] = bv.y[q];
+    bv.pz[k] = bv.z[q];
+  }
+}
+
+// A rejected rescale undone: molecule m's sites and primaries restored.
+__global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
+                                                       const VolItem* items) {
+  const BatchBox& B = table[items[blockIdx.y].slot];
+  const BoxView& bv = B.bv;
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= bv.n_mol) return;
+  const int s0 = bv.mol_start[m], s1 = s0 + bv.mol_len[m];
+  for (int s = s0; s < s1; s++) {
+    bv.x[s] = B.bx[s];
+    bv.y[s] = B.by[s];
+    bv.z[s] = B.bz[s];
+  }
+  const int ps = bv.mol_pidx_start[m], pe = ps + bv.mol_pidx_count[m];
+  for (int k = ps; k < pe; k++) {
+    const int q = bv.pidx[k];
+    bv.px[k] = bv.x[q];
+    bv.py[k] = bv.y[q];
+    bv.pz[k] = bv.z[q];
+  }
+}
+
+// Cell lists of every listed box in cell mode, rebuilt for its current
+// geometry: clear, assign, order (see cells.h).
+__global__ __launch_bounds__(256) void k_cells_clear_multi(const BatchBox* table,
+                                                           const VolItem* items) {
+  const BoxView& bv = table[items[blockIdx.y].slot].bv;
+  if (bv.mode != MCGPU_NB_CELL) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < cells_total(bv.cells)) bv.cells.count[c] = 0;
+  if (c == 0) bv.cells.overflow[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_cells_assign_multi(const BatchBox* table,
+                                                            const VolItem* items) {
+  const BoxView& bv = table[items[blockIdx.y].slot].bv;
+  if (bv.mode != MCGPU_NB_CELL) return;
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m < bv.n_mol) cells_assign_one(bv, m);
+}
+
+__global__ __launch_bounds__(256) void k_cells_sort_multi(const BatchBox* table,
+                                                          const VolItem* items) {
+  const BoxView& bv = table[items[blockIdx.y].slot].bv;
+  if (bv.mode != MCGPU_NB_CELL) return;
+  const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (c < cells_total(bv.cells)) cells_sort_wave(bv.cells, c);
 }
 
 // ------------------------------------------------------------ host side
@@ -579,9 +720,12 @@ struct mcgpu_batch {
   BoxCtl* h_ctl = nullptr;      // pinned mirror
   Rng31* d_rng = nullptr;
   Rng31* h_rng = nullptr;       // pinned mirror
-  int32_t* d_active = nullptr;  // [2 * max_boxes]: group 0, then group 1
-  int32_t* h_active = nullptr;  // pinned
+  int32_t* d_active = nullptr;  // [2 * (max_boxes + 1)]: group 0, then group 1,
+  int32_t* h_active = nullptr;  // pinned     each [count, slots...]
   double* h_vol = nullptr;      // pinned [4 * max_boxes] volume-move energies
+  VolItem* d_items = nullptr;   // [max_boxes] list of a batched NPT pass
+  VolItem* h_items = nullptr;   // pinned
+  double* d_volout = nullptr;   // [2 * max_boxes] energies of a batched pass
   long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
   double clock_khz = 100000.0;
   bool pending = false;         // a launch whose results were not consumed
@@ -632,29 +776,71 @@ int slot_energy_async(mcgpu_batch* b, int s, int which) {
                               (double*)(base + S.lay.total) + 2 * which, b->stream);
 }
 
-// Cell grid for slot s's current box; the all-molecule scan when the grid
-// no longer fits the allocation.  Capacity overflow is detected on the
-// device (the kernels then scan every molecule).
-int slot_rebuild_cells(mcgpu_batch* b, int s) {
+// Cell geometry of slot s's current box into its descriptor (host copy):
+// cell mode when the grid fits the allocation, else the all-molecule scan.
+// Returns the cell count (0 without cells).
+int slot_set_geometry(mcgpu_batch* b, int s) {
   BatchSlot& S = b->slots[s];
   BoxView& v = b->h_table[s].bv;
   if (S.lay.ncell == 0) {
     v.mode = MCGPU_NB_BRUTE;
-    return MCGPU_OK;
+    return 0;
   }
   int n[3];
   double iw[3];
   cell_geometry(v.L, std::sqrt(v.cutoff2), S.spread, n, iw);
   if (n[0] * n[1] * n[2] > S.lay.ncell) {
     v.mode = MCGPU_NB_BRUTE;
-    return MCGPU_OK;
+    return 0;
   }
   for (int k = 0; k < 3; k++) {
     v.cells.n[k] = n[k];
     v.cells.inv_w[k] = iw[k];
   }
   v.mode = MCGPU_NB_CELL;
-  return launch_cells_build(v, b->stream);
+  return n[0] * n[1] * n[2];
+}
+
+// Cell lists of slot s for its current box (stream-ordered).  Capacity
+// overflow is detected on the device (the kernels then scan every molecule).
+int slot_rebuild_cells(mcgpu_batch* b, int s) {
+  if (slot_set_geometry(b, s) == 0) return MCGPU_OK;
+  return launch_cells_build(b->h_table[s].bv, b->stream);
+}
+
+int upload_table(mcgpu_batch* b) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table, b->h_table, sizeof(BatchBox) * b->max_boxes,
+                               hipMemcpyHostToDevice, b->stream));
+  return MCGPU_OK;
+}
+
+// The batched passes over the first k entries of h_items (copied to the
+// device first): energies into d_volout[2 i + which].
+int launch_energy_pass(mcgpu_batch* b, int k, int max_mol, int which) {
+  hipLaunchKernelGGL(k_energy_multi<64>, dim3(std::min(max_mol, 1024), k), dim3(64), 0,
+                     b->stream, b->d_table, b->d_items);
+  hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, b->stream, b->d_table,
+                     b->d_items, which, b->d_volout);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int launch_cells_pass(mcgpu_batch* b, int k, int max_mol, int max_cells) {
+  if (max_cells == 0) return MCGPU_OK;
+  hipLaunchKernelGGL(k_cells_clear_multi, dim3((max_cells + 255) / 256, k), dim3(256), 0,
+                     b->stream, b->d_table, b->d_items);
+  hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0,
+                     b->stream, b->d_table, b->d_items);
+  hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0,
+                     b->stream, b->d_table, b->d_items);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int upload_items(mcgpu_batch* b, int k) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * k,
+                               hipMemcpyHostToDevice, b->stream));
+  return MCGPU_OK;
 }
 
 // NPT volume moves of every pending box, batched: all rescales and energies
@@ -669,36 +855,51 @@ int slot_rebuild_cells(mcgpu_batch* b, int s) {
 // u1 and u2 are the box's next two draws of its own RNG stream.
 int batch_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
-  for (int s : pend) {
+  const int K = (int)pend.size();
+  int max_mol = 0, max_cells = 0;
+  for (int k = 0; k < K; k++) {
+    const int s = pend[k];
     BatchSlot& S = b->slots[s];
     BoxView& v = b->h_table[s].bv;
-    char* base = slot_base(b, s);
     Rng31& g = b->h_rng[s];
     const double u1 = random_real(rng31_next(g), 0.0, 1.0);
     S.u_accept = random_real(rng31_next(g), 0.0, 1.0);
-    for (int k = 0; k < 3; k++) S.L_old[k] = v.L[k];
+    for (int d = 0; d < 3; d++) S.L_old[d] = v.L[d];
     S.v_old = v.L[0] * v.L[1] * v.L[2];
     S.v_new = S.v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
-    const double f = std::cbrt(S.v_new / S.v_old);
-    if ((rc = slot_energy_async(b, s, 0))) return rc;
-    const size_t nb = 8 * (size_t)v.n_site;
-    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bx, v.x, nb, hipMemcpyDeviceToDevice, b->stream));
-    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.by, v.y, nb, hipMemcpyDeviceToDevice, b->stream));
-    MCGPU_HIP_TRY(hipMemcpyAsync(base + S.lay.bz, v.z, nb, hipMemcpyDeviceToDevice, b->stream));
-    if ((rc = launch_volume_scale(v, f, b->stream))) return rc;
-    for (int k = 0; k < 3; k++) v.L[k] = S.L_old[k] * f;
-    if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
-    if ((rc = slot_rebuild_cells(b, s))) return rc;
-    if ((rc = slot_energy_async(b, s, 1))) return rc;
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->h_vol + 4 * s, base + S.lay.total, 32,
-                                 hipMemcpyDeviceToHost, b->stream));
+    b->h_items[k] = VolItem{s, 0, std::cbrt(S.v_new / S.v_old)};
+    max_mol = std::max(max_mol, v.n_mol);
   }
+  // energies at the current volume, then every box rescaled, its cells
+  // rebuilt for the new geometry, and the energies at the new volume
+  if ((rc = upload_items(b, K)) || (rc = launch_energy_pass(b, K, max_mol, 0))) return rc;
+  for (int k = 0; k < K; k++) {
+    const int s = pend[k];
+    BoxView& v = b->h_table[s].bv;
+    for (int d = 0; d < 3; d++) v.L[d] = b->slots[s].L_old[d] * b->h_items[k].f;
+    max_cells = std::max(max_cells, slot_set_geometry(b, s));
+  }
+  if ((rc = upload_table(b))) return rc;
+  hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, b->stream,
+                     b->d_table, b->d_items);
+  MCGPU_HIP_TRY(hipGetLastError());
+  if ((rc = launch_cells_pass(b, K, max_mol, max_cells)) ||
+      (rc = launch_energy_pass(b, K, max_mol, 1)))
+    return rc;
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_vol, b->d_volout, sizeof(double) * 2 * K,
+                               hipMemcpyDeviceToHost, b->stream));
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  for (int s : pend) {
+  // decisions; rejected boxes are listed again for the undo
+  int nrej = 0, rej_mol = 0, rej_cells = 0;
+  std::vector<double> e(2 * K);
+  std::memcpy(e.data(), b->h_vol, sizeof(double) * 2 * K);
+  std::vector<VolItem> items(b->h_items, b->h_items + K);
+  for (int k = 0; k < K; k++) {
+    const int s = pend[k];
     BatchSlot& S = b->slots[s];
     BoxView& v = b->h_table[s].bv;
     BoxCtl& c = b->h_ctl[s];
-    const double e_old = b->h_vol[4 * s], e_new = b->h_vol[4 * s + 2];
+    const double e_old = e[2 * k], e_new = e[2 * k + 1];
     const double lrc_new = S.lrc * S.v_old / S.v_new;
     const double de = (e_new - e_old) + (lrc_new - S.lrc);
     double prob = v.n_mol * (std::log(S.v_new) - std::log(S.v_old));
@@ -710,24 +911,26 @@ int batch_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
       c.energy += de;
       S.lrc = lrc_new;
     } else {
-      char* base = slot_base(b, s);
-      hipLaunchKernelGGL(k_restore_coords, dim3((v.n_site + 255) / 256), dim3(256), 0,
-                         b->stream, v.x, v.y, v.z, (const double*)(base + S.lay.bx),
-                         (const double*)(base + S.lay.by), (const double*)(base + S.lay.bz),
-                         v.n_site);
-      MCGPU_HIP_TRY(hipGetLastError());
-      for (int k = 0; k < 3; k++) v.L[k] = S.L_old[k];
-      if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream))) return rc;
-      if ((rc = slot_rebuild_cells(b, s))) return rc;
+      for (int d = 0; d < 3; d++) v.L[d] = S.L_old[d];
+      rej_cells = std::max(rej_cells, slot_set_geometry(b, s));
+      rej_mol = std::max(rej_mol, v.n_mol);
+      b->h_items[nrej++] = items[k];
     }
     c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
     c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
-    if ((rc = upload_slot(b, s))) return rc;
-    if ((rc = upload_ctl(b, s))) return rc;
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_rng + s, b->h_rng + s, sizeof(Rng31),
-                                 hipMemcpyHostToDevice, b->stream));
   }
-  b->version++;
+  if (nrej) {
+    if ((rc = upload_items(b, nrej)) || (rc = upload_table(b))) return rc;
+    hipLaunchKernelGGL(k_restore_multi, dim3((rej_mol + 255) / 256, nrej), dim3(256), 0,
+                       b->stream, b->d_table, b->d_items);
+    MCGPU_HIP_TRY(hipGetLastError());
+    if ((rc = launch_cells_pass(b, nrej, rej_mol, rej_cells))) return rc;
+  }
+  // the host mirrors are current for every box between launches
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * b->max_boxes,
+                               hipMemcpyHostToDevice, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_rng, b->h_rng, sizeof(Rng31) * b->max_boxes,
+                               hipMemcpyHostToDevice, b->stream));
   return MCGPU_OK;
 }
 
@@ -762,7 +965,6 @@ void free_slot(mcgpu_batch* b, int s) {
   if (!b->slots[s].used) return;
   b->arena.release(b->slots[s].off);
   b->slots[s] = BatchSlot();
-  b->version++;
 }
 
 }  // namespace
@@ -804,12 +1006,15 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
       hipMalloc(&b->d_table, sizeof(BatchBox) * M) != hipSuccess ||
       hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) != hipSuccess ||
       hipMalloc(&b->d_rng, sizeof(Rng31) * M) != hipSuccess ||
-      hipMalloc(&b->d_active, sizeof(int32_t) * 2 * M) != hipSuccess ||
+      hipMalloc(&b->d_active, sizeof(int32_t) * 2 * (M + 1)) != hipSuccess ||
       hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * 2 * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_vol, sizeof(double) * 4 * M, 0) != hipSuccess) {
+      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * 2 * (M + 1), 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_vol, sizeof(double) * 4 * M, 0) != hipSuccess ||
+      hipMalloc(&b->d_items, sizeof(VolItem) * M) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) != hipSuccess ||
+      hipMalloc(&b->d_volout, sizeof(double) * 2 * M) != hipSuccess) {
     set_error("mcgpu_batch_create: allocation failed");
     return fail(MCGPU_ERR_NOMEM);
   }
@@ -840,6 +1045,9 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (b->h_rng) (void)hipHostFree(b->h_rng);
   if (b->h_active) (void)hipHostFree(b->h_active);
   if (b->h_vol) (void)hipHostFree(b->h_vol);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->h_items) (void)hipHostFree(b->h_items);
+  if (b->d_volout) (void)hipFree(b->d_volout);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -958,6 +1166,11 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.max_r = d->max_rotate;
   E.precision = p->precision;
   E.record = p->record_trace ? 1 : 0;
+  E.partial = (double*)(base + L.partial);
+  E.total = (double*)(base + L.total);
+  E.bx = (double*)(base + L.bx);
+  E.by = (double*)(base + L.by);
+  E.bz = (double*)(base + L.bz);
   E.tr_mol = (int32_t*)(base + L.trm);
   E.tr_de = (double*)(base + L.trd);
   E.tr_acc = (int8_t*)(base + L.tra);
@@ -1018,7 +1231,6 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
                      (const double*)(base + L.total), p->energy_offset);
   MCGPU_HIP_TRY(hipGetLastError());
   b->stale_ctl = true;  // the device now holds the start energy
-  b->version++;
   *slot_out = s;
   return MCGPU_OK;
 }
@@ -1041,46 +1253,52 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   if (!b || sweeps < 1) return MCGPU_ERR_INVALID;
   int rc = batch_sync(b);
   if (rc) return rc;
+  const int stride = b->max_boxes + 1;
   int n[2] = {0, 0};
   for (int s = 0; s < b->max_boxes; s++)
     if (b->slots[s].used && b->h_ctl[s].status == kRunning) {
       const int g = b->slots[s].group;
-      b->h_active[g * b->max_boxes + n[g]++] = s;
+      b->h_active[g * stride + 1 + n[g]++] = s;
     }
   if (n[0] + n[1] == 0) return MCGPU_OK;
+  for (int g = 0; g < 2; g++) {
+    b->h_active[g * stride] = n[g];
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, b->h_active + g * stride,
+                                 sizeof(int32_t) * (1 + n[g]), hipMemcpyHostToDevice,
+                                 b->stream));
+  }
+  // grids rounded up (surplus work-groups exit at once), so the captured
+  // graph survives most admissions and retirements
+  int grid[2];
+  for (int g = 0; g < 2; g++) grid[g] = n[g] ? std::min(b->max_boxes, (n[g] + 31) & ~31) : 0;
   const bool same = b->gexec && b->graph_version == b->version &&
-                    b->graph_sweeps == sweeps && b->n_active[0] == n[0] &&
-                    b->n_active[1] == n[1];
+                    b->graph_sweeps == sweeps && b->n_active[0] == grid[0] &&
+                    b->n_active[1] == grid[1];
   if (!same) {
     if (b->gexec) MCGPU_HIP_TRY(hipGraphExecDestroy(b->gexec));
     if (b->graph) MCGPU_HIP_TRY(hipGraphDestroy(b->graph));
     b->gexec = nullptr;
     b->graph = nullptr;
-    for (int g = 0; g < 2; g++)
-      if (n[g])
-        MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * b->max_boxes,
-                                     b->h_active + g * b->max_boxes, sizeof(int32_t) * n[g],
-                                     hipMemcpyHostToDevice, b->stream));
     // One tick = the persistent launch of each work-group-size group, the
-    // two running side by side; captured once and replayed while the box
-    // set and sweep count are unchanged.
+    // two running side by side; captured once and replayed while the grid
+    // sizes, sweep count and time budget are unchanged.
     MCGPU_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipEventRecord(b->fork, b->stream);
-    hipStreamWaitEvent(b->stream2, b->fork, 0);
-    if (n[0])
-      hipLaunchKernelGGL(k_batch<256>, dim3(n[0]), dim3(256), 0, b->stream, b->d_table,
+    MCGPU_HIP_TRY(hipEventRecord(b->fork, b->stream));
+    MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
+    if (grid[0])
+      hipLaunchKernelGGL(k_batch<256>, dim3(grid[0]), dim3(256), 0, b->stream, b->d_table,
                          b->d_ctl, b->d_active, sweeps, b->budget_ticks);
-    if (n[1])
-      hipLaunchKernelGGL(k_batch<512>, dim3(n[1]), dim3(512), 0, b->stream2, b->d_table,
-                         b->d_ctl, b->d_active + b->max_boxes, sweeps, b->budget_ticks);
-    hipEventRecord(b->join, b->stream2);
-    hipStreamWaitEvent(b->stream, b->join, 0);
+    if (grid[1])
+      hipLaunchKernelGGL(k_batch<512>, dim3(grid[1]), dim3(512), 0, b->stream2, b->d_table,
+                         b->d_ctl, b->d_active + stride, sweeps, b->budget_ticks);
+    MCGPU_HIP_TRY(hipEventRecord(b->join, b->stream2));
+    MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
     MCGPU_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
     MCGPU_HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->graph_version = b->version;
     b->graph_sweeps = sweeps;
-    b->n_active[0] = n[0];
-    b->n_active[1] = n[1];
+    b->n_active[0] = grid[0];
+    b->n_active[1] = grid[1];
   }
   MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));

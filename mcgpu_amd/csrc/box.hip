@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "box_internal.h"
+#include "cells.h"
 #include "eval.h"
 #include "fastmath.h"
 
@@ -97,39 +98,13 @@ __global__ void k_cells_clear(CellGrid g, int ncell) {
 
 __global__ void k_cells_assign(BoxView bv) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= bv.n_mol) return;
-  const CellGrid& g = bv.cells;
-  const int p = bv.mol_pidx_start[m];
-  const int c = cell_index(g, cell_coord(bv.px[p], g.inv_w[0], g.n[0]),
-                           cell_coord(bv.py[p], g.inv_w[1], g.n[1]),
-                           cell_coord(bv.pz[p], g.inv_w[2], g.n[2]));
-  const int slot = atomicAdd(&g.count[c], 1);
-  g.mol_cell[m] = c;
-  if (slot < g.cap) {
-    g.slots[c * g.cap + slot] = m;
-  } else {
-    atomicOr(g.overflow, 1);
-  }
+  if (m < bv.n_mol) cells_assign_one(bv, m);
 }
 
-// Sort each cell's slots by molecule id so the candidate order (and so the
-// summation order) does not depend on atomic arrival order.
-__global__ void k_cells_sort(CellGrid g, int ncell) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int n = g.count[c];
-  n = n < g.cap ? n : g.cap;
-  int32_t* s = g.slots + (size_t)c * g.cap;
-  for (int i = 1; i < n; i++) {
-    const int v = s[i];
-    int k = i - 1;
-    while (k >= 0 && s[k] > v) {
-      s[k + 1] = s[k];
-      k--;
-    }
-    s[k + 1] = v;
-  }
-  for (int i = 0; i < n; i++) g.mol_slot[s[i]] = i;
+// One wave per cell (4 cells per 256-lane work-group).
+__global__ __launch_bounds__(256) void k_cells_sort(CellGrid g, int ncell) {
+  const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (c < ncell) cells_sort_wave(g, c);
 }
 
 // Verlet list of molecule i: every j whose first primary lies within rv of
@@ -249,8 +224,7 @@ int launch_cells_build(const BoxView& bv, hipStream_t s) {
                      g, ncell);
   hipLaunchKernelGGL(k_cells_assign, dim3((bv.n_mol + 255) / 256), dim3(256), 0,
                      s, bv);
-  hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 255) / 256), dim3(256), 0, s,
-                     g, ncell);
+  hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 3) / 4), dim3(256), 0, s, g, ncell);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
@@ -468,8 +442,8 @@ int build_cells(mcgpu_box* b) {
                                  hipMemcpyDeviceToHost, b->stream));
     MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
     if (!ovf) {
-      hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 255) / 256), dim3(256), 0,
-                         b->stream, g, ncell);
+      hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 3) / 4), dim3(256), 0, b->stream,
+                         g, ncell);
       MCGPU_HIP_TRY(hipGetLastError());
       return MCGPU_OK;
     }
