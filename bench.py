@@ -312,7 +312,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_batch (one graph launch: k_batch<256> and k_batch<512> on two streams)",
+            "kernel": "k_batch (one graph launch: 256-, 512- and 1024-lane instances on three streams)",
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": st.kernel_ms_total / max(st.ticks, 1),
         },

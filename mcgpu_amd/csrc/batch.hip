@@ -166,8 +166,8 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
     xo = bv.x[start + lane];
     yo = bv.y[start + lane];
     zo = bv.z[start + lane];
-    sh.q64[lane] = bv.p64[param + lane];
-    sh.q32[lane] = bv.p32[param + lane];
+    sh.q64[lane] = sh.prm64[param + lane];
+    sh.q32[lane] = sh.prm32[param + lane];
   }
   if (npri > 1 && lane < npri) pof = bv.pidx[pstart + lane] - start;
   // the rotation's draws while the sites load
@@ -361,6 +361,8 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   }
   const BoxView& bv = S.tb.bv;
   const bool mixed = S.tb.precision == MCGPU_MIXED;
+  load_params<NT>(bv, S.ev);
+  __syncthreads();
   for (int s = 0; s < sweeps; s++) {
     if (w0) front_move<NT>(S, g, s, budget, t_start);
     __syncthreads();
@@ -424,6 +426,7 @@ __global__ __launch_bounds__(NT) void k_energy_multi(const BatchBox* table,
   const BatchBox& B = table[items[blockIdx.y].slot];
   const BoxView& bv = B.bv;
   const bool mixed = B.precision == MCGPU_MIXED;
+  load_params<NT>(bv, sh);
   for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
     load_molecule<NT>(bv, mol, sh);
     __syncthreads();
@@ -670,23 +673,29 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   return L;
 }
 
-// Work-group size for a box: 512 lanes when one move has enough candidate
-// and pair work to keep them busy, else 256 (the estimate uses the
-// molecule density inside the cutoff sphere).
+// Work-group size for a box, from the per-move work estimated with the
+// molecule density inside the cutoff sphere: candidates scanned plus pair
+// lane-steps.  256 lanes for light moves, 512 for medium, 1024 (16 waves,
+// which hide the latency of the candidate loads and keep the FP64 units fed
+// during the pair sums) for heavy ones; measured per-move latency on MI355X
+// with one box per CU: water, 256 molecules, all-pairs scan 7.6 / 7.0 /
+// 7.4 us (256 / 512 / 1024 lanes); water with cells 12.7 / 10.3 / 9.9 us;
+// 16-site bicyclic with cells 23.8 / 17.9 / 16.6 us.
+constexpr int kGroups = 3;
+constexpr int kGroupNT[kGroups] = {256, 512, 1024};
+
 int choose_group(const mcgpu_box_desc& d, int neighbor, int max_len) {
   const double V = d.box[0] * d.box[1] * d.box[2];
   const double rho = d.n_molecules / V;
   const double rc = d.cutoff;
   const double hits = std::min<double>(d.n_molecules - 1, 4.18879 * rc * rc * rc * rho);
   double cand = d.n_molecules;
-  if (neighbor == MCGPU_NEIGHBOR_CELL) cand = std::min<double>(cand, 27.0 * rc * rc * rc * rho * 1.3);
+  if (neighbor == MCGPU_NEIGHBOR_CELL) cand = std::min<double>(cand, 27.0 * rc * rc * rc * rho);
   int aw = 1;
   while (aw < max_len) aw *= 2;
-  const double lane_steps = hits * max_len * aw;
-  return (cand + lane_steps / 2 > 3000.0) ? 1 : 0;
+  const double work = cand + hits * max_len * aw / 2;
+  return work > 1000.0 ? 2 : (work > 600.0 ? 1 : 0);
 }
-
-constexpr int kGroupNT[2] = {256, 512};
 
 }  // namespace mcgpu
 
@@ -709,8 +718,9 @@ struct BatchSlot {
 
 struct mcgpu_batch {
   hipStream_t stream = nullptr;  // main stream (group 0 and all setup)
-  hipStream_t stream2 = nullptr; // group 1 during a tick
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr, join = nullptr;
+  hipStream_t side[kGroups - 1] = {};  // groups 1.. during a tick
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
+  hipEvent_t join[kGroups - 1] = {};
   Arena arena;
   int32_t max_boxes = 0;
   std::vector<BatchSlot> slots;
@@ -720,8 +730,8 @@ struct mcgpu_batch {
   BoxCtl* h_ctl = nullptr;      // pinned mirror
   Rng31* d_rng = nullptr;
   Rng31* h_rng = nullptr;       // pinned mirror
-  int32_t* d_active = nullptr;  // [2 * (max_boxes + 1)]: group 0, then group 1,
-  int32_t* h_active = nullptr;  // pinned     each [count, slots...]
+  int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
+  int32_t* h_active = nullptr;  // pinned     [count, slots...]
   double* h_vol = nullptr;      // pinned [4 * max_boxes] volume-move energies
   VolItem* d_items = nullptr;   // [max_boxes] list of a batched NPT pass
   VolItem* h_items = nullptr;   // pinned
@@ -730,7 +740,7 @@ struct mcgpu_batch {
   double clock_khz = 100000.0;
   bool pending = false;         // a launch whose results were not consumed
   bool stale_ctl = false;       // h_ctl older than device-side changes
-  int n_active[2] = {0, 0};
+  int n_active[kGroups] = {};
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -999,18 +1009,20 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
     b->clock_khz = khz;
   const size_t M = max_boxes;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
       hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&b->d_table, sizeof(BatchBox) * M) != hipSuccess ||
       hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) != hipSuccess ||
       hipMalloc(&b->d_rng, sizeof(Rng31) * M) != hipSuccess ||
-      hipMalloc(&b->d_active, sizeof(int32_t) * 2 * (M + 1)) != hipSuccess ||
+      hipMalloc(&b->d_active, sizeof(int32_t) * kGroups * (M + 1)) != hipSuccess ||
       hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * 2 * (M + 1), 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * kGroups * (M + 1), 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_vol, sizeof(double) * 4 * M, 0) != hipSuccess ||
       hipMalloc(&b->d_items, sizeof(VolItem) * M) != hipSuccess ||
       hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) != hipSuccess ||
@@ -1033,7 +1045,8 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
 extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (!b) return MCGPU_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  for (hipStream_t s : b->side)
+    if (s) (void)hipStreamSynchronize(s);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_table) (void)hipFree(b->d_table);
@@ -1051,9 +1064,11 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->fork) (void)hipEventDestroy(b->fork);
-  if (b->join) (void)hipEventDestroy(b->join);
+  for (hipEvent_t e : b->join)
+    if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
-  if (b->stream2) (void)hipStreamDestroy(b->stream2);
+  for (hipStream_t s : b->side)
+    if (s) (void)hipStreamDestroy(s);
   delete b;
   return MCGPU_OK;
 }
@@ -1107,6 +1122,11 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
     return MCGPU_ERR_NOMEM;
   }
   const ParamTable pt = build_param_table(*d);
+  if ((int)pt.p64.size() > kParamCap) {
+    set_error("mcgpu_batch_add: more than " + std::to_string(kParamCap) +
+              " distinct site-parameter entries");
+    return MCGPU_ERR_INVALID;
+  }
   const BoxLayout L = plan_layout(*d, *p, pt);
   size_t off = 0;
   if (!b->arena.alloc(L.bytes, &off)) {
@@ -1149,6 +1169,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   v.uni_len = pt.uni_len;
   v.uni_param = pt.uni_param;
   v.uni_p0 = pt.uni_p0;
+  v.n_param = (int32_t)pt.p64.size();
   v.mode = MCGPU_NB_BRUTE;
   for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
   v.cutoff2 = d->cutoff * d->cutoff;
@@ -1254,14 +1275,16 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   int rc = batch_sync(b);
   if (rc) return rc;
   const int stride = b->max_boxes + 1;
-  int n[2] = {0, 0};
+  int n[kGroups] = {};
   for (int s = 0; s < b->max_boxes; s++)
     if (b->slots[s].used && b->h_ctl[s].status == kRunning) {
       const int g = b->slots[s].group;
       b->h_active[g * stride + 1 + n[g]++] = s;
     }
-  if (n[0] + n[1] == 0) return MCGPU_OK;
-  for (int g = 0; g < 2; g++) {
+  int total = 0;
+  for (int g = 0; g < kGroups; g++) total += n[g];
+  if (total == 0) return MCGPU_OK;
+  for (int g = 0; g < kGroups; g++) {
     b->h_active[g * stride] = n[g];
     MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, b->h_active + g * stride,
                                  sizeof(int32_t) * (1 + n[g]), hipMemcpyHostToDevice,
@@ -1269,36 +1292,42 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   }
   // grids rounded up (surplus work-groups exit at once), so the captured
   // graph survives most admissions and retirements
-  int grid[2];
-  for (int g = 0; g < 2; g++) grid[g] = n[g] ? std::min(b->max_boxes, (n[g] + 31) & ~31) : 0;
-  const bool same = b->gexec && b->graph_version == b->version &&
-                    b->graph_sweeps == sweeps && b->n_active[0] == grid[0] &&
-                    b->n_active[1] == grid[1];
+  int grid[kGroups];
+  bool same = b->gexec && b->graph_version == b->version && b->graph_sweeps == sweeps;
+  for (int g = 0; g < kGroups; g++) {
+    grid[g] = n[g] ? std::min(b->max_boxes, (n[g] + 31) & ~31) : 0;
+    same = same && b->n_active[g] == grid[g];
+  }
   if (!same) {
     if (b->gexec) MCGPU_HIP_TRY(hipGraphExecDestroy(b->gexec));
     if (b->graph) MCGPU_HIP_TRY(hipGraphDestroy(b->graph));
     b->gexec = nullptr;
     b->graph = nullptr;
-    // One tick = the persistent launch of each work-group-size group, the
-    // two running side by side; captured once and replayed while the grid
-    // sizes, sweep count and time budget are unchanged.
+    // One tick = the persistent launch of each work-group-size group, side
+    // by side on their own streams; captured once and replayed while the
+    // grid sizes, sweep count and time budget are unchanged.
     MCGPU_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     MCGPU_HIP_TRY(hipEventRecord(b->fork, b->stream));
-    MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
+    for (int g = 1; g < kGroups; g++)
+      MCGPU_HIP_TRY(hipStreamWaitEvent(b->side[g - 1], b->fork, 0));
     if (grid[0])
       hipLaunchKernelGGL(k_batch<256>, dim3(grid[0]), dim3(256), 0, b->stream, b->d_table,
                          b->d_ctl, b->d_active, sweeps, b->budget_ticks);
     if (grid[1])
-      hipLaunchKernelGGL(k_batch<512>, dim3(grid[1]), dim3(512), 0, b->stream2, b->d_table,
+      hipLaunchKernelGGL(k_batch<512>, dim3(grid[1]), dim3(512), 0, b->side[0], b->d_table,
                          b->d_ctl, b->d_active + stride, sweeps, b->budget_ticks);
-    MCGPU_HIP_TRY(hipEventRecord(b->join, b->stream2));
-    MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
+    if (grid[2])
+      hipLaunchKernelGGL(k_batch<1024>, dim3(grid[2]), dim3(1024), 0, b->side[1], b->d_table,
+                         b->d_ctl, b->d_active + 2 * stride, sweeps, b->budget_ticks);
+    for (int g = 1; g < kGroups; g++) {
+      MCGPU_HIP_TRY(hipEventRecord(b->join[g - 1], b->side[g - 1]));
+      MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->join[g - 1], 0));
+    }
     MCGPU_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
     MCGPU_HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->graph_version = b->version;
     b->graph_sweeps = sweeps;
-    b->n_active[0] = grid[0];
-    b->n_active[1] = grid[1];
+    for (int g = 0; g < kGroups; g++) b->n_active[g] = grid[g];
   }
   MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));

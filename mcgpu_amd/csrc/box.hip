@@ -37,6 +37,7 @@ __global__ __launch_bounds__(NT) void k_move(BoxView bv, MoveArgs a,
                                              MoveOut* out) {
   __shared__ EvalShared<NT> sh;
   const int t = threadIdx.x;
+  load_params<NT>(bv, sh);
   load_molecule<NT>(bv, a.mol, sh);
   const int len = bv.mol_len[a.mol];
   if (a.has_new) {
@@ -149,6 +150,7 @@ template <int NT, bool MIXED>
 __global__ __launch_bounds__(NT) void k_system(BoxView bv, double* partial,
                                                double* pairs) {
   __shared__ EvalShared<NT> sh;
+  load_params<NT>(bv, sh);
   for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
     load_molecule<NT>(bv, mol, sh);
     __syncthreads();
@@ -294,12 +296,12 @@ ParamTable build_param_table(const mcgpu_box_desc& d) {
         const double sg = d.sigma[s0 + k], ep = d.epsilon[s0 + k];
         const bool real = sg >= 0 && ep >= 0;  // BruteForceStep.cpp:64-65
         SiteParam64 p;
-        p.ss = real ? std::sqrt(sg) : 0.0;
-        p.se = real ? std::sqrt(ep) : 0.0;
-        p.q = real ? d.charge[s0 + k] : 0.0;
+        p.s = real ? sg : 0.0;
+        p.e = real ? 2.0 * std::sqrt(ep) : 0.0;
+        p.q = real ? d.charge[s0 + k] * std::sqrt(kCoulombK) : 0.0;
         p.real = real ? 1.0 : 0.0;
         t.p64.push_back(p);
-        t.p32.push_back(SiteParam32{(float)p.ss, (float)p.se, (float)p.q, (float)p.real});
+        t.p32.push_back(SiteParam32{(float)p.s, (float)p.e, (float)p.q, (float)p.real});
       }
       it = sig.emplace(key, base).first;
     }
@@ -563,12 +565,17 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
     set_error("mcgpu_box_create: precision must be MCGPU_FP64 or MCGPU_MIXED");
     return MCGPU_ERR_INVALID;
   }
+  const ParamTable pt = build_param_table(*d);
+  if ((int)pt.p64.size() > kParamCap) {
+    set_error("mcgpu_box_create: more than " + std::to_string(kParamCap) +
+              " distinct site-parameter entries");
+    return MCGPU_ERR_INVALID;
+  }
   int rc = ensure_device();
   if (rc) return rc;
   auto* b = new (std::nothrow) mcgpu_box();
   if (!b) return MCGPU_ERR_NOMEM;
   b->precision = precision;
-  const ParamTable pt = build_param_table(*d);
   b->spread = pt.spread;
   b->n_pidx = d->n_primary;
   b->h_mol_start.assign(d->mol_start, d->mol_start + d->n_molecules);
@@ -580,6 +587,7 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   v.uni_len = pt.uni_len;
   v.uni_param = pt.uni_param;
   v.uni_p0 = pt.uni_p0;
+  v.n_param = (int32_t)pt.p64.size();
   v.mode = MCGPU_NB_BRUTE;
   for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
   v.cutoff2 = d->cutoff * d->cutoff;

@@ -24,12 +24,18 @@ namespace mcgpu {
 constexpr int kMaxSites = 32;        // MCGPU_MAX_MOL_SITES
 constexpr double kCoulombK = 332.06; // SimulationStep.cpp:364
 
+// Site parameters in the form the pair terms use: s = sigma, e =
+// 2 sqrt(epsilon), q = charge * sqrt(332.06), real = 1 (0 for dummy sites,
+// whose other fields are 0).  Then sigma_ab^2 = s_a s_b, 4 eps_ab = e_a e_b
+// and 332.06 q_a q_b = q_a q_b (SimulationStep.cpp:344-375 with its
+// geometric mixing sigma_ab = sqrt(|sigma_a sigma_b|), eps_ab likewise).
 struct SiteParam64 {
-  double ss, se, q, real;
+  double s, e, q, real;
 };
 struct SiteParam32 {
-  float ss, se, q, real;
+  float s, e, q, real;
 };
+constexpr int kParamCap = 128;  // distinct site-parameter entries per box
 
 // Per-molecule record read with one 16-byte load: first site, site count,
 // offset of the first primary site within the molecule, first entry in the
@@ -98,6 +104,8 @@ struct BoxView {
   int32_t uni_len;  // 0 when the box is not uniform
   int32_t uni_param;
   int32_t uni_p0;
+  int32_t n_param;  // entries of the site-parameter tables
+  int32_t pad0;
   double L[3];
   double cutoff2;
   double kT;
@@ -107,11 +115,14 @@ struct BoxView {
 
 // ---------------------------------------------------------------- math
 
-// SimulationStep.cpp:377-385: one wrap per dimension.
+// SimulationStep.cpp:377-385: one wrap per dimension, d + L when d < -L/2,
+// d - L when d > L/2 (the second test can only fire when the first did
+// not), as one FMA with k in {1, 0, -1}: k L is exact, so the sum rounds
+// exactly as the reference's addition.
 __device__ __forceinline__ double wrap1(double d, double L) {
-  d += (d < -0.5 * L) ? L : 0.0;
-  d -= (d > 0.5 * L) ? L : 0.0;
-  return d;
+  const double h = 0.5 * L;
+  const double k = d < -h ? 1.0 : (d > h ? -1.0 : 0.0);
+  return fma(k, L, d);
 }
 
 // 1/sqrt(x) in FP64: the hardware estimate (v_rsq_f64) refined by two
@@ -126,21 +137,19 @@ __device__ __forceinline__ double rsqrt64(double x) {
 }
 
 // FP64 LJ + Coulomb of one site pair from the wrapped displacement
-// (SimulationStep.cpp:344-367; BruteForceStep.cpp:64-73).  Callers skip
-// dummy sites; r^2 == 0 contributes nothing.
+// (SimulationStep.cpp:344-367; BruteForceStep.cpp:64-73): with s2 =
+// sigma_ab^2 / r^2, LJ = 4 eps_ab (s2^6 - s2^3) and Coulomb = 332.06 q_a q_b
+// / r.  r^2 == 0 contributes nothing; dummy sites (all-zero parameters)
+// contribute 0.
 __device__ __forceinline__ double pair_energy64(double dx, double dy, double dz,
-                                                const SiteParam64& a,
-                                                const SiteParam64& b) {
+                                                double sa, double ea, double qa,
+                                                double sb, double eb, double qb) {
   const double r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 == 0.0) return 0.0;
   const double rinv = rsqrt64(r2);
-  const double inv_r2 = rinv * rinv;
-  const double sig = a.ss * b.ss;
-  const double s2 = sig * sig * inv_r2;
+  const double s2 = (sa * sb) * (rinv * rinv);
   const double s6 = s2 * s2 * s2;
-  const double lj = 4.0 * (a.se * b.se) * (s6 * s6 - s6);
-  const double coul = (a.q * b.q * kCoulombK) * rinv;
-  return lj + coul;
+  const double e = (ea * eb) * fma(s6, s6, -s6) + (qa * qb) * rinv;
+  return r2 == 0.0 ? 0.0 : e;
 }
 
 // Mixed precision: displacement wrapped in FP64, pair terms in FP32, the
@@ -150,32 +159,27 @@ __device__ __forceinline__ double pair_energy64(double dx, double dy, double dz,
 constexpr float kNearContactS6 = 4.0f;
 
 __device__ __forceinline__ double pair_energy32(double dx, double dy, double dz,
-                                                const SiteParam32& a,
-                                                const SiteParam32& b,
-                                                const SiteParam64& a64,
-                                                const SiteParam64* b64) {
+                                                float sa, float ea, float qa,
+                                                float sb, float eb, float qb,
+                                                double sa64, double ea64,
+                                                double sb64, double eb64) {
   const double r2d = dx * dx + dy * dy + dz * dz;
   const float r2 = (float)r2d;
-  if (r2 == 0.0f) return 0.0;
   float rinv = __builtin_amdgcn_rsqf(r2);
   rinv = rinv * (1.5f - 0.5f * r2 * rinv * rinv);  // one Newton step
-  const float inv_r2 = rinv * rinv;
-  const float sig = a.ss * b.ss;
-  const float s2 = sig * sig * inv_r2;
+  const float s2 = (sa * sb) * (rinv * rinv);
   const float s6 = s2 * s2 * s2;
-  const float eps = a.se * b.se;
-  const float coul = (a.q * b.q * 332.06f) * rinv;
+  const float eps = ea * eb;
+  const float coul = (qa * qb) * rinv;
   double lj;
   if (s6 > kNearContactS6 && eps != 0.0f) {
-    const SiteParam64 bb = *b64;
-    const double sd = a64.ss * bb.ss;
-    const double s2d = sd * sd / r2d;
+    const double s2d = (sa64 * sb64) / r2d;
     const double s6d = s2d * s2d * s2d;
-    lj = 4.0 * (a64.se * bb.se) * (s6d * s6d - s6d);
+    lj = (ea64 * eb64) * (s6d * s6d - s6d);
   } else {
-    lj = (double)(4.0f * eps * (s6 * s6 - s6));
+    lj = (double)(eps * (s6 * s6 - s6));
   }
-  return lj + (double)coul;
+  return r2d == 0.0 ? 0.0 : lj + (double)coul;
 }
 
 // ------------------------------------------------------- wave / block ops

@@ -26,6 +26,7 @@ template <int NT>
 struct EvalShared {
   static constexpr int kCap = NT * 2 < 256 ? 256 : NT * 2;
   static constexpr int kItemCap = 2048;  // items with a cell entry in item_cell
+  static constexpr int kSiteCap = NT >= 256 ? 768 : 192;  // staged hit sites
   static constexpr int kNW = NT / 64;
   double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
   double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
@@ -39,6 +40,13 @@ struct EvalShared {
   int32_t cell_id[64];       // cells of the current range (at most 4x4x4)
   int32_t cell_pre[65];      // prefix of their occupancies
   uint8_t item_cell[kItemCap];  // range cell of each candidate item
+  // sites of a chunk of hits, staged for the pair sums: coordinates and the
+  // site-parameter entry (-1: no such site in that hit)
+  double sx[kSiteCap], sy[kSiteCap], sz[kSiteCap];
+  int32_t sp[kSiteCap];
+  // the box's site-parameter table
+  SiteParam64 prm64[kParamCap];
+  SiteParam32 prm32[kParamCap];
   double red[kNW];
   double red2[kNW];
   unsigned long long redc[kNW];  // sites << 32 | candidates, per wave
@@ -56,6 +64,17 @@ struct EvalShared {
 
 __device__ __forceinline__ int lane_group_log2(int len) {
   return len > 1 ? 32 - __clz(len - 1) : 0;
+}
+
+// Copies the box's site-parameter table into LDS (all threads; the caller
+// places a barrier before it is read).  Tables are at most kParamCap entries
+// (checked when a box is created).
+template <int NT>
+__device__ __forceinline__ void load_params(const BoxView& bv, EvalShared<NT>& sh) {
+  for (int k = threadIdx.x; k < bv.n_param; k += NT) {
+    sh.prm64[k] = bv.p64[k];
+    sh.prm32[k] = bv.p32[k];
+  }
 }
 
 // Loads molecule `mol`'s current sites (and parameters) into LDS.
@@ -108,95 +127,106 @@ __device__ __forceinline__ bool primary_in_range(const BoxView& bv, int ps, int 
   return in;
 }
 
-// Sums the pair energies of the compacted hits into e_old / e_new.  Each
-// lane handles two (hit, site b) items per iteration so the site loads of
-// both are in flight together.  No barrier: callers order LDS reuse.
+// floor(u / m) for u < 65536, 1 <= m <= 32, with magic = 2^32 / m + 1.
+__device__ __forceinline__ int div_small(int u, unsigned long long magic) {
+  return (int)(((unsigned long long)(unsigned)u * magic) >> 32);
+}
+
+// Sums the pair energies of the compacted hits into e_old / e_new, a chunk
+// of hits at a time: all lanes first stage the chunk's sites from the box
+// into LDS (one round trip to memory), then lanes are grouped Aw at a time,
+// a group takes one (hit, site b) item and each lane of it one site a of the
+// moving molecule.
 template <int NT, bool MIXED>
 __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
-                                           int nhits, double& e_old,
-                                           double& e_new, unsigned& npairs,
-                                           unsigned& nsites) {
+                                           int nhits, double& e_old, double& e_new,
+                                           unsigned& npairs, unsigned& nsites) {
+  constexpr int CAP = EvalShared<NT>::kSiteCap;
   const int M = bv.max_len;
+  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
   const int len = sh.len;
   const int lg = sh.aw_log2;
   const int a = threadIdx.x & ((1 << lg) - 1);
+  const int ac = a < len ? a : 0;
   const int G = NT >> lg;
-  const bool lane_real = a < len && sh.q64[a < len ? a : 0].real != 0.0;
-  const int total = nhits * M;
-  for (int i0 = threadIdx.x >> lg; i0 < total; i0 += 2 * G) {
-    int site[2], pi[2], f[2];
-    bool ok[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int item = i0 + u * G;
-      ok[u] = false;
-      site[u] = pi[u] = f[u] = 0;
-      if (item < total) {
-        const int k = item / M;
-        const int b = item - k * M;
-        const int lf = sh.cand_lf[k];
-        if (b < (lf & 0xff)) {
-          ok[u] = true;
-          site[u] = sh.cand_start[k] + b;
-          pi[u] = sh.cand_param[k] + b;
-          f[u] = lf >> 8;
-        }
+  const bool lane_real = a < len && sh.q64[ac].real != 0.0;
+  // this lane's site a: old and proposed position, parameters
+  const double xo = sh.ox[ac], yo = sh.oy[ac], zo = sh.oz[ac];
+  const double xn = sh.nx[ac], yn = sh.ny[ac], zn = sh.nz[ac];
+  const double sa = sh.q64[ac].s, ea = sh.q64[ac].e, qa = sh.q64[ac].q;
+  const float sa32 = sh.q32[ac].s, ea32 = sh.q32[ac].e, qa32 = sh.q32[ac].q;
+  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
+  const int chunk = CAP / M;
+  for (int h0 = 0; h0 < nhits; h0 += chunk) {
+    const int nh = nhits - h0 < chunk ? nhits - h0 : chunk;
+    const int tot = nh * M;
+    for (int u = threadIdx.x; u < tot; u += NT) {
+      const int kk = div_small(u, magic);
+      const int b = u - kk * M;
+      const int k = h0 + kk;
+      const int lf = sh.cand_lf[k];
+      int pi = -1;
+      if (b < (lf & 0xff)) {
+        const int site = sh.cand_start[k] + b;
+        sh.sx[u] = bv.x[site];
+        sh.sy[u] = bv.y[site];
+        sh.sz[u] = bv.z[site];
+        pi = sh.cand_param[k] + b;
       }
-      nsites += (ok[u] && a == 0);
-      ok[u] = ok[u] && lane_real;
+      sh.sp[u] = pi;
     }
-    double xb[2], yb[2], zb[2];
-    if (MIXED) {
-      SiteParam32 pb[2];
+    __syncthreads();
+    // two items per step for ILP; 1024-lane groups hide latency with
+    // their 16 waves instead and need the registers
+    constexpr int U = NT >= 1024 ? 1 : 2;
+    for (int i0 = threadIdx.x >> lg; i0 < tot; i0 += U * G) {
+      int pi[U], f[U];
+      double xb[U], yb[U], zb[U];
 #pragma unroll
-      for (int u = 0; u < 2; u++)
-        if (ok[u]) {
-          xb[u] = bv.x[site[u]];
-          yb[u] = bv.y[site[u]];
-          zb[u] = bv.z[site[u]];
-          pb[u] = bv.p32[pi[u]];
+      for (int v = 0; v < U; v++) {
+        const int i = i0 + v * G;
+        pi[v] = -1;
+        f[v] = 0;
+        if (i < tot) {
+          pi[v] = sh.sp[i];
+          f[v] = sh.cand_lf[h0 + div_small(i, magic)] >> 8;
+          xb[v] = sh.sx[i];
+          yb[v] = sh.sy[i];
+          zb[v] = sh.sz[i];
         }
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        if (!ok[u] || pb[u].real == 0.0f) continue;
-        if (f[u] & 1) {
-          e_old += pair_energy32(wrap1(xb[u] - sh.ox[a], bv.L[0]),
-                                 wrap1(yb[u] - sh.oy[a], bv.L[1]),
-                                 wrap1(zb[u] - sh.oz[a], bv.L[2]), sh.q32[a], pb[u],
-                                 sh.q64[a], bv.p64 + pi[u]);
-          npairs++;
-        }
-        if (f[u] & 2)
-          e_new += pair_energy32(wrap1(xb[u] - sh.nx[a], bv.L[0]),
-                                 wrap1(yb[u] - sh.ny[a], bv.L[1]),
-                                 wrap1(zb[u] - sh.nz[a], bv.L[2]), sh.q32[a], pb[u],
-                                 sh.q64[a], bv.p64 + pi[u]);
+        nsites += (pi[v] >= 0 && a == 0);
+        if (!lane_real) pi[v] = -1;
       }
-    } else {
-      SiteParam64 pb[2];
 #pragma unroll
-      for (int u = 0; u < 2; u++)
-        if (ok[u]) {
-          xb[u] = bv.x[site[u]];
-          yb[u] = bv.y[site[u]];
-          zb[u] = bv.z[site[u]];
-          pb[u] = bv.p64[pi[u]];
+      for (int v = 0; v < U; v++) {
+        if (pi[v] < 0) continue;
+        if (MIXED) {
+          const SiteParam32 pb = sh.prm32[pi[v]];
+          const SiteParam64& pd = sh.prm64[pi[v]];
+          if (f[v] & 1) {
+            e_old += pair_energy32(wrap1(xb[v] - xo, L0), wrap1(yb[v] - yo, L1),
+                                   wrap1(zb[v] - zo, L2), sa32, ea32, qa32, pb.s, pb.e, pb.q,
+                                   sa, ea, pd.s, pd.e);
+            npairs++;
+          }
+          if (f[v] & 2)
+            e_new += pair_energy32(wrap1(xb[v] - xn, L0), wrap1(yb[v] - yn, L1),
+                                   wrap1(zb[v] - zn, L2), sa32, ea32, qa32, pb.s, pb.e, pb.q,
+                                   sa, ea, pd.s, pd.e);
+        } else {
+          const SiteParam64 pb = sh.prm64[pi[v]];
+          if (f[v] & 1) {
+            e_old += pair_energy64(wrap1(xb[v] - xo, L0), wrap1(yb[v] - yo, L1),
+                                   wrap1(zb[v] - zo, L2), sa, ea, qa, pb.s, pb.e, pb.q);
+            npairs++;
+          }
+          if (f[v] & 2)
+            e_new += pair_energy64(wrap1(xb[v] - xn, L0), wrap1(yb[v] - yn, L1),
+                                   wrap1(zb[v] - zn, L2), sa, ea, qa, pb.s, pb.e, pb.q);
         }
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        if (!ok[u] || pb[u].real == 0.0) continue;
-        if (f[u] & 1) {
-          e_old += pair_energy64(wrap1(xb[u] - sh.ox[a], bv.L[0]),
-                                 wrap1(yb[u] - sh.oy[a], bv.L[1]),
-                                 wrap1(zb[u] - sh.oz[a], bv.L[2]), sh.q64[a], pb[u]);
-          npairs++;
-        }
-        if (f[u] & 2)
-          e_new += pair_energy64(wrap1(xb[u] - sh.nx[a], bv.L[0]),
-                                 wrap1(yb[u] - sh.ny[a], bv.L[1]),
-                                 wrap1(zb[u] - sh.nz[a], bv.L[2]), sh.q64[a], pb[u]);
       }
     }
+    __syncthreads();
   }
 }
 
@@ -326,8 +356,8 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
     }
     par ^= 1;
     if (nhits + total > EvalShared<NT>::kCap) {  // block-uniform
+      // (ends with a barrier: every lane is done with the hits before reuse)
       pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs, nsites);
-      __syncthreads();  // every lane done with the hits before they are reused
       nhits = 0;
     }
     if (f) {

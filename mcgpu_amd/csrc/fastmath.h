@@ -22,33 +22,52 @@ constexpr double kLn2A = 0.6931471801362932;
 constexpr double kLn2B = 4.236521365809284e-10;
 constexpr double kInvLn2 = 1.4426950408889634;
 
+// Coefficient tables in constant memory.  The functions read them through a
+// pointer the optimiser cannot see through, so the coefficients are loaded
+// (scalar loads) where they are used rather than held in registers across
+// the caller's loops, where they would crowd out the loop's own values.
+__constant__ double kSinCosTab[24] = {
+    kTwoOverPi, kPio2A, kPio2B, kPio2C,
+    // sin: 1/17!, -1/15!, 1/13!, -1/11!, 1/9!, -1/7!, 1/5!, -1/3!
+    2.8114572543455206e-15, -7.647163731819816e-13, 1.6059043836821613e-10,
+    -2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
+    0.008333333333333333, -0.16666666666666666,
+    // cos: -1/18!, 1/16!, -1/14!, 1/12!, -1/10!, 1/8!, -1/6!, 1/4!, -1/2!
+    -1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
+    2.08767569878681e-09, -2.755731922398589e-07, 2.48015873015873e-05,
+    -0.001388888888888889, 0.041666666666666664, -0.5, 0.0};
+__constant__ double kExpTab[20] = {
+    kInvLn2, kLn2A, kLn2B, -745.2, 709.8,
+    // 1/13! ... 1/2!, 1, 1
+    1.6059043836821613e-10, 2.08767569878681e-09, 2.505210838544172e-08,
+    2.755731922398589e-07, 2.7557319223985893e-06, 2.48015873015873e-05,
+    0.0001984126984126984, 0.001388888888888889, 0.008333333333333333,
+    0.041666666666666664, 0.16666666666666666, 0.5, 1.0, 1.0, 0.0};
+
+typedef const __attribute__((address_space(4))) double* ConstTab;
+
+__device__ __forceinline__ ConstTab opaque_tab(ConstTab p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // sin and cos of x for |x| < 2^21 (trial-move angles are within +-2 pi).
 __device__ __forceinline__ void sincos_fp64(double x, double& s, double& c) {
-  const double k = rint(x * kTwoOverPi);
-  double r = fma(-k, kPio2A, x);
-  r = fma(-k, kPio2B, r);
-  r = fma(-k, kPio2C, r);  // |r| <= pi/4
+  const ConstTab T = opaque_tab((ConstTab)kSinCosTab);
+  const double k = rint(x * T[0]);
+  double r = fma(-k, T[1], x);
+  r = fma(-k, T[2], r);
+  r = fma(-k, T[3], r);  // |r| <= pi/4
   const double r2 = r * r;
-  // sin r = r (1 - r^2/3! + r^4/5! - ... - r^16/17!)... as r + r^3 P(r^2)
-  double ps = 2.8114572543455206e-15;        //  1/17!
-  ps = fma(ps, r2, -7.647163731819816e-13);  // -1/15!
-  ps = fma(ps, r2, 1.6059043836821613e-10);  //  1/13!
-  ps = fma(ps, r2, -2.505210838544172e-08);  // -1/11!
-  ps = fma(ps, r2, 2.7557319223985893e-06);  //  1/9!
-  ps = fma(ps, r2, -0.0001984126984126984);  // -1/7!
-  ps = fma(ps, r2, 0.008333333333333333);    //  1/5!
-  ps = fma(ps, r2, -0.16666666666666666);    // -1/3!
+  // sin r = r + r^3 P(r^2), Taylor terms to r^17
+  double ps = T[4];
+#pragma unroll
+  for (int i = 5; i < 12; i++) ps = fma(ps, r2, T[i]);
   const double sr = fma(ps * r2, r, r);
-  // cos r = 1 - r^2/2! + ... + r^18/18! as 1 + r^2 Q(r^2)
-  double pc = -1.5619206968586225e-16;       // -1/18!
-  pc = fma(pc, r2, 4.779477332387385e-14);   //  1/16!
-  pc = fma(pc, r2, -1.1470745597729725e-11); // -1/14!
-  pc = fma(pc, r2, 2.08767569878681e-09);    //  1/12!
-  pc = fma(pc, r2, -2.755731922398589e-07);  // -1/10!
-  pc = fma(pc, r2, 2.48015873015873e-05);    //  1/8!
-  pc = fma(pc, r2, -0.001388888888888889);   // -1/6!
-  pc = fma(pc, r2, 0.041666666666666664);    //  1/4!
-  pc = fma(pc, r2, -0.5);                    // -1/2!
+  // cos r = 1 + r^2 Q(r^2), Taylor terms to r^18
+  double pc = T[12];
+#pragma unroll
+  for (int i = 13; i < 21; i++) pc = fma(pc, r2, T[i]);
   const double cr = fma(pc, r2, 1.0);
   const int q = ((int)k) & 3;
   s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
@@ -57,26 +76,16 @@ __device__ __forceinline__ void sincos_fp64(double x, double& s, double& c) {
 
 // e^x for finite x (NaN passes through, underflows to 0, overflows to inf).
 __device__ __forceinline__ double exp_fp64(double x) {
+  const ConstTab T = opaque_tab((ConstTab)kExpTab);
   if (!(x == x)) return x;
-  if (x < -745.2) return 0.0;
-  if (x > 709.8) return __builtin_huge_val();
-  const double k = rint(x * kInvLn2);
-  double r = fma(-k, kLn2A, x);
-  r = fma(-k, kLn2B, r);  // |r| <= ln2/2
-  double p = 1.6059043836821613e-10;  // 1/13!
-  p = fma(p, r, 2.08767569878681e-09);   // 1/12!
-  p = fma(p, r, 2.505210838544172e-08);  // 1/11!
-  p = fma(p, r, 2.755731922398589e-07);  // 1/10!
-  p = fma(p, r, 2.7557319223985893e-06); // 1/9!
-  p = fma(p, r, 2.48015873015873e-05);   // 1/8!
-  p = fma(p, r, 0.0001984126984126984);  // 1/7!
-  p = fma(p, r, 0.001388888888888889);   // 1/6!
-  p = fma(p, r, 0.008333333333333333);   // 1/5!
-  p = fma(p, r, 0.041666666666666664);   // 1/4!
-  p = fma(p, r, 0.16666666666666666);    // 1/3!
-  p = fma(p, r, 0.5);                    // 1/2!
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
+  if (x < T[3]) return 0.0;
+  if (x > T[4]) return __builtin_huge_val();
+  const double k = rint(x * T[0]);
+  double r = fma(-k, T[1], x);
+  r = fma(-k, T[2], r);  // |r| <= ln2/2
+  double p = T[5];
+#pragma unroll
+  for (int i = 6; i < 19; i++) p = fma(p, r, T[i]);
   return ldexp(p, (int)k);
 }
 
