@@ -417,31 +417,206 @@ __global__ void k_restore_coords(double* x, double* y, double* z,
 // One launch covers every box of a list (blockIdx.y = list entry), so the
 // volume moves of all boxes due in a tick cost a handful of launches.
 
-// Full-system energy (SimulationStep.cpp:40-71): work-group x of entry y sums
-// E_mol(mol) over partners j > mol for molecules x, x + gridDim.x, ...
-template <int NT>
-__global__ __launch_bounds__(NT) void k_energy_multi(const BatchBox* table,
-                                                     const VolItem* items) {
-  __shared__ EvalShared<NT> sh;
+// Full-system energy (SimulationStep.cpp:40-71: the sum over i of E_i(i),
+// partners j > i) of every listed box, one wave per molecule i: the lanes
+// scan the candidates (the cells around i's first primary, or all j > i),
+// test the primary cutoff (SimulationStep.cpp:317-330), compact the hits
+// into the wave's LDS list by ballot, then sum the site pairs of the hits
+// (each lane one (hit, site b) item against every site a of i), and the
+// wave sum goes to partial[i].  Waves never wait for each other: a
+// throughput kernel that keeps many waves per CU in flight.
+constexpr int kWaveHits = 256;
+
+struct WaveEval {
+  double xa[kMaxSites], ya[kMaxSites], za[kMaxSites];  // sites of molecule i
+  int32_t pa[kMaxSites];                               // their parameter entries
+  int32_t hit_start[kWaveHits];
+  int32_t hit_param[kWaveHits];
+  int32_t hit_len[kWaveHits];
+};
+
+template <bool MIXED>
+__device__ __forceinline__ double wave_hits_energy(const BoxView& bv, const WaveEval& W,
+                                                   int len_i, int nh,
+                                                   const SiteParam64* P64,
+                                                   const SiteParam32* P32) {
+  const int M = bv.max_len;
+  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
+  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
+  double e = 0.0;
+  for (int u = wlane(); u < nh * M; u += 64) {
+    const int k = div_small(u, magic);
+    const int b = u - k * M;
+    if (b >= W.hit_len[k]) continue;
+    const int sb = W.hit_start[k] + b;
+    const int pb = W.hit_param[k] + b;
+    const SiteParam64 B = P64[pb];
+    if (B.real == 0.0) continue;
+    const double xb = bv.x[sb], yb = bv.y[sb], zb = bv.z[sb];
+    for (int a = 0; a < len_i; a++) {
+      const int pa = W.pa[a];
+      const SiteParam64 A = P64[pa];
+      if (A.real == 0.0) continue;
+      const double dx = wrap1(xb - W.xa[a], L0);
+      const double dy = wrap1(yb - W.ya[a], L1);
+      const double dz = wrap1(zb - W.za[a], L2);
+      if (MIXED) {
+        const SiteParam32 A32 = P32[pa], B32 = P32[pb];
+        e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e, B.s,
+                           B.e);
+      } else {
+        e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
+      }
+    }
+  }
+  return e;
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEval& W, int i,
+                                                       const SiteParam64* P64,
+                                                       const SiteParam32* P32) {
+  const int lane = wlane();
+  int start, len, param, psi, pci;
+  if (bv.uni_len > 0) {
+    len = bv.uni_len;
+    start = i * len;
+    param = bv.uni_param;
+    psi = i;
+    pci = 1;
+  } else {
+    const MolMeta m = bv.meta[i];
+    start = m.start;
+    len = m.len;
+    param = m.param;
+    psi = bv.single_primary ? i : m.pstart;
+    pci = bv.single_primary ? 1 : bv.mol_pidx_count[i];
+  }
+  if (lane < len) {
+    W.xa[lane] = bv.x[start + lane];
+    W.ya[lane] = bv.y[start + lane];
+    W.za[lane] = bv.z[start + lane];
+    W.pa[lane] = param + lane;
+  }
+  // the primaries of i (at most kMaxSites) as lane values
+  double pix = 0.0, piy = 0.0, piz = 0.0;
+  if (lane < pci) {
+    pix = bv.px[psi + lane];
+    piy = bv.py[psi + lane];
+    piz = bv.pz[psi + lane];
+  }
+  __builtin_amdgcn_wave_barrier();
+  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
+  // candidate source
+  const CellGrid& g = bv.cells;
+  const bool cells = bv.mode == MCGPU_NB_CELL && g.overflow[0] == 0;
+  CellRange r{};
+  int ncell = 0, incl = 0, nitems = bv.n_mol - (i + 1);
+  if (cells) {
+    int co[3];
+    co[0] = cell_coord(readlane_f64(pix, 0), g.inv_w[0], g.n[0]);
+    co[1] = cell_coord(readlane_f64(piy, 0), g.inv_w[1], g.n[1]);
+    co[2] = cell_coord(readlane_f64(piz, 0), g.inv_w[2], g.n[2]);
+    r = make_range(g, co, co);  // at most 3 x 3 x 3 cells
+    ncell = r.total();
+    int n = 0;
+    if (lane < ncell) {
+      n = g.count[r.cell(g, lane)];
+      n = n < g.cap ? n : g.cap;
+    }
+    incl = wave_incl_scan_i32(n);
+    nitems = readlane_i32(incl, 63);
+  }
+  double e = 0.0;
+  int nh = 0;
+  for (int base = 0; base < nitems; base += 64) {
+    const int item = base + lane;
+    int j = -1;
+    if (cells) {
+      // cell holding `item`: the number of cells whose inclusive prefix is
+      // <= item (all lanes take part in the lane reads)
+      int lo = 0;
+      for (int step = 32; step > 0; step >>= 1) {
+        const int probe = lo + step - 1;
+        const int v = __shfl(incl, probe < 63 ? probe : 63);
+        if (probe < ncell && v <= item) lo += step;
+      }
+      const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
+      if (item < nitems) j = g.slots[r.cell(g, lo) * g.cap + (item - (lo > 0 ? before : 0))];
+    } else if (item < nitems) {
+      j = i + 1 + item;
+    }
+    bool hit = false;
+    int hs = 0, hp = 0, hl = 0;
+    if (j > i) {
+      const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
+      const int pcj = bv.single_primary ? 1 : bv.mol_pidx_count[j];
+      for (int q = 0; q < pcj; q++) {
+        const double qx = bv.px[psj + q], qy = bv.py[psj + q], qz = bv.pz[psj + q];
+        for (int p = 0; p < pci; p++) {
+          const double dx = wrap1(qx - readlane_f64(pix, p), L0);
+          const double dy = wrap1(qy - readlane_f64(piy, p), L1);
+          const double dz = wrap1(qz - readlane_f64(piz, p), L2);
+          hit |= dx * dx + dy * dy + dz * dz <= bv.cutoff2;
+        }
+      }
+      if (hit) {
+        if (bv.uni_len > 0) {
+          hl = bv.uni_len;
+          hs = j * hl;
+          hp = bv.uni_param;
+        } else {
+          const MolMeta m = bv.meta[j];
+          hs = m.start;
+          hl = m.len;
+          hp = m.param;
+        }
+      }
+    }
+    const unsigned long long bal = __ballot(hit);
+    const int cnt = __popcll(bal);
+    if (nh + cnt > kWaveHits) {  // wave-uniform
+      e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32);
+      __builtin_amdgcn_wave_barrier();
+      nh = 0;
+    }
+    if (hit) {
+      const int pos = nh + wave_rank(bal);
+      W.hit_start[pos] = hs;
+      W.hit_param[pos] = hp;
+      W.hit_len[pos] = hl;
+    }
+    nh += cnt;
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32);
+  __builtin_amdgcn_wave_barrier();
+  return wave_sum_f64(e);
+}
+
+__global__ __launch_bounds__(256) void k_energy_waves(const BatchBox* table,
+                                                      const VolItem* items) {
+  __shared__ WaveEval ws[4];
+  __shared__ SiteParam64 p64[kParamCap];
+  __shared__ SiteParam32 p32[kParamCap];
   const BatchBox& B = table[items[blockIdx.y].slot];
   const BoxView& bv = B.bv;
+  for (int k = threadIdx.x; k < bv.n_param; k += 256) {
+    p64[k] = bv.p64[k];
+    p32[k] = bv.p32[k];
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
   const bool mixed = B.precision == MCGPU_MIXED;
-  load_params<NT>(bv, sh);
-  for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
-    load_molecule<NT>(bv, mol, sh);
-    __syncthreads();
-    double eo, en;
-    if (mixed)
-      eval_molecule<NT, true>(bv, mol, mol + 1, false, sh, eo, en, nullptr);
-    else
-      eval_molecule<NT, false>(bv, mol, mol + 1, false, sh, eo, en, nullptr);
-    if (threadIdx.x == 0) B.partial[mol] = eo;
-    __syncthreads();
+  for (int i = blockIdx.x * 4 + w; i < bv.n_mol; i += gridDim.x * 4) {
+    const double e = mixed ? wave_molecule_energy<true>(bv, ws[w], i, p64, p32)
+                           : wave_molecule_energy<false>(bv, ws[w], i, p64, p32);
+    if (wlane() == 0) B.partial[i] = e;
   }
 }
 
-// Fixed-order sum of entry blockIdx.x's partials into total[which] and
-// out[2 * entry + which].
+// Fixed-order sum of entry blockIdx.x's per-molecule partials into
+// total[which] and out[2 * entry + which].
 __global__ __launch_bounds__(256) void k_reduce_multi(const BatchBox* table,
                                                       const VolItem* items, int which,
                                                       double* out) {
@@ -656,7 +831,7 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   L.covf = take(4);
   L.vstale = take(4);
   L.vcount = take(4);
-  L.partial = take(8 * nm);
+  L.partial = take(8 * std::max(ns, nm));  // per-site partial energies
   L.pairs = take(8 * nm);
   L.total = take(32);  // [E_pairs, pairs] twice (before / after a rescale)
   if (p.ensemble == MCGPU_NPT) {
@@ -712,15 +887,31 @@ struct BatchSlot {
   double spread = 0.0; // largest primary-to-first-primary distance
   int64_t vol_moves = 0, vol_acc = 0;
   int fault = 0;
+  bool in_volume = false;  // a volume move is in flight (not in ticks)
+  bool dirty = false;      // the device's control block is newer than h_ctl
   // volume move in flight
   double v_old = 0.0, v_new = 0.0, u_accept = 0.0, L_old[3] = {0, 0, 0};
 };
 
+// Host side of a batch.  Streams: `stream` carries the ticks (group 0 of the
+// graph), the read-back after them and, between ticks, the set-up work
+// (admissions, NPT volume passes and their undo); `side[]` the other
+// work-group-size groups of a tick.  vstream and sstream name the streams
+// of volume passes and set-up work; both are `stream`: work beside a tick
+// delays the start of the tick's persistent work-groups by more than it
+// saves (measured), so it runs between ticks.
+// Ownership: a box's control block and generator state belong to the device
+// from its launch in a tick until that tick's read-back is merged into the
+// host mirrors; a box in a volume pass or being set up belongs to the host.
 struct mcgpu_batch {
-  hipStream_t stream = nullptr;  // main stream (group 0 and all setup)
-  hipStream_t side[kGroups - 1] = {};  // groups 1.. during a tick
+  hipStream_t stream = nullptr;
+  hipStream_t side[kGroups - 1] = {};
+  hipStream_t vstream = nullptr;
+  hipStream_t sstream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipEvent_t join[kGroups - 1] = {};
+  hipEvent_t ev_vol = nullptr;    // a volume pass's energies are on the host
+  hipEvent_t ev_ready = nullptr;  // set-up work queued so far is done
   Arena arena;
   int32_t max_boxes = 0;
   std::vector<BatchSlot> slots;
@@ -728,18 +919,25 @@ struct mcgpu_batch {
   BatchBox* h_table = nullptr;  // pinned mirror (the host's copy)
   BoxCtl* d_ctl = nullptr;
   BoxCtl* h_ctl = nullptr;      // pinned mirror
+  BoxCtl* h_ctl_rb = nullptr;   // pinned read-back of a tick
   Rng31* d_rng = nullptr;
   Rng31* h_rng = nullptr;       // pinned mirror
+  Rng31* h_rng_rb = nullptr;    // pinned read-back of a tick
   int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
   int32_t* h_active = nullptr;  // pinned     [count, slots...]
-  double* h_vol = nullptr;      // pinned [4 * max_boxes] volume-move energies
-  VolItem* d_items = nullptr;   // [max_boxes] list of a batched NPT pass
+  double* h_vol = nullptr;      // pinned [2 * max_boxes] volume-pass energies
+  VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
-  double* d_volout = nullptr;   // [2 * max_boxes] energies of a batched pass
+  VolItem* d_undo = nullptr;    // [max_boxes] rejected rescales to undo
+  VolItem* h_undo = nullptr;    // pinned
+  VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
+  double* d_volout = nullptr;   // [2 * max_boxes] energies of a volume pass
   long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
   double clock_khz = 100000.0;
-  bool pending = false;         // a launch whose results were not consumed
-  bool stale_ctl = false;       // h_ctl older than device-side changes
+  bool pending = false;         // a tick in flight
+  bool vol_inflight = false;    // a volume pass in flight
+  std::vector<int> launched;    // slots of the tick in flight
+  std::vector<int> vol_list;    // slots of the volume pass in flight
   int n_active[kGroups] = {};
   // graph cache
   hipGraph_t graph = nullptr;
@@ -754,37 +952,25 @@ struct mcgpu_batch {
 
 namespace {
 
-int upload_slot(mcgpu_batch* b, int s) {
+int upload_slot(mcgpu_batch* b, int s, hipStream_t st) {
   MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table + s, b->h_table + s, sizeof(BatchBox),
-                               hipMemcpyHostToDevice, b->stream));
+                               hipMemcpyHostToDevice, st));
   return MCGPU_OK;
 }
 
-int upload_ctl(mcgpu_batch* b, int s) {
+int upload_ctl(mcgpu_batch* b, int s, hipStream_t st) {
   MCGPU_HIP_TRY(hipMemcpyAsync(b->d_ctl + s, b->h_ctl + s, sizeof(BoxCtl),
-                               hipMemcpyHostToDevice, b->stream));
+                               hipMemcpyHostToDevice, st));
   return MCGPU_OK;
 }
 
-int readback(mcgpu_batch* b) {
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
-                               hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng, b->d_rng, sizeof(Rng31) * b->max_boxes,
-                               hipMemcpyDeviceToHost, b->stream));
+int upload_rng(mcgpu_batch* b, int s, hipStream_t st) {
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_rng + s, b->h_rng + s, sizeof(Rng31),
+                               hipMemcpyHostToDevice, st));
   return MCGPU_OK;
 }
 
 char* slot_base(mcgpu_batch* b, int s) { return b->arena.base() + b->slots[s].off; }
-
-// Full-system energy of slot s into total[which] (stream-ordered).
-int slot_energy_async(mcgpu_batch* b, int s, int which) {
-  BatchSlot& S = b->slots[s];
-  char* base = slot_base(b, s);
-  return launch_system_energy(b->h_table[s].bv, b->h_table[s].precision,
-                              (double*)(base + S.lay.partial),
-                              (double*)(base + S.lay.pairs),
-                              (double*)(base + S.lay.total) + 2 * which, b->stream);
-}
 
 // Cell geometry of slot s's current box into its descriptor (host copy):
 // cell mode when the grid fits the allocation, else the all-molecule scan.
@@ -811,62 +997,47 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
   return n[0] * n[1] * n[2];
 }
 
-// Cell lists of slot s for its current box (stream-ordered).  Capacity
-// overflow is detected on the device (the kernels then scan every molecule).
-int slot_rebuild_cells(mcgpu_batch* b, int s) {
-  if (slot_set_geometry(b, s) == 0) return MCGPU_OK;
-  return launch_cells_build(b->h_table[s].bv, b->stream);
-}
-
-int upload_table(mcgpu_batch* b) {
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table, b->h_table, sizeof(BatchBox) * b->max_boxes,
-                               hipMemcpyHostToDevice, b->stream));
-  return MCGPU_OK;
-}
-
-// The batched passes over the first k entries of h_items (copied to the
-// device first): energies into d_volout[2 i + which].
-int launch_energy_pass(mcgpu_batch* b, int k, int max_mol, int which) {
-  hipLaunchKernelGGL(k_energy_multi<64>, dim3(std::min(max_mol, 1024), k), dim3(64), 0,
-                     b->stream, b->d_table, b->d_items);
-  hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, b->stream, b->d_table,
-                     b->d_items, which, b->d_volout);
+// The batched passes over the first k entries of a device list: full
+// energies into d_volout[2 i + which]; cell lists for the current geometry.
+int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
+                       int which, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_waves, dim3(std::min((max_mol + 3) / 4, 2048), k), dim3(256), 0,
+                     st, b->d_table, items);
+  hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, st, b->d_table, items, which,
+                     b->d_volout);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
 
-int launch_cells_pass(mcgpu_batch* b, int k, int max_mol, int max_cells) {
+int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
+                      int max_cells, hipStream_t st) {
   if (max_cells == 0) return MCGPU_OK;
-  hipLaunchKernelGGL(k_cells_clear_multi, dim3((max_cells + 255) / 256, k), dim3(256), 0,
-                     b->stream, b->d_table, b->d_items);
-  hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0,
-                     b->stream, b->d_table, b->d_items);
-  hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0,
-                     b->stream, b->d_table, b->d_items);
+  hipLaunchKernelGGL(k_cells_clear_multi, dim3((max_cells + 255) / 256, k), dim3(256), 0, st,
+                     b->d_table, items);
+  hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0, st,
+                     b->d_table, items);
+  hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0, st,
+                     b->d_table, items);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
 
-int upload_items(mcgpu_batch* b, int k) {
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * k,
-                               hipMemcpyHostToDevice, b->stream));
-  return MCGPU_OK;
-}
-
-// NPT volume moves of every pending box, batched: all rescales and energies
-// are queued, one synchronisation collects them, the decisions are made and
-// rejected moves undone, and the updated descriptors are queued for upload.
+// NPT volume moves of the boxes in `pend`, batched: every box rescaled, its
+// cells rebuilt and its full energy at the new volume computed, all queued
+// on the volume stream (the boxes sit out the ticks until the decision).
 // Semantics (the build's own; the reference's NPT path, Ensemble.cpp:9-28 /
 // SimulationStep.cpp:476-518, always grows the box by 1.1 and never
 // recomputes the energy):
 //   V' = V (1 + delta (2 u1 - 1)); molecules translated so first primaries
-//   scale by (V'/V)^(1/3); dE = dE_pairs + LRC(V') - LRC(V);
-//   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2.
+//   scale by (V'/V)^(1/3); E' = E_pairs(V') + LRC(V'); dE = E' - E, with E
+//   the box's running energy (start energy plus accepted move dE);
+//   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2, and then the
+//   running energy becomes E' exactly.
 // u1 and u2 are the box's next two draws of its own RNG stream.
-int batch_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
+int start_volume_pass(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
   const int K = (int)pend.size();
-  int max_mol = 0, max_cells = 0;
+  int max_mol = 0, max_site = 0, max_cells = 0;
   for (int k = 0; k < K; k++) {
     const int s = pend[k];
     BatchSlot& S = b->slots[s];
@@ -879,83 +1050,98 @@ int batch_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
     S.v_new = S.v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
     b->h_items[k] = VolItem{s, 0, std::cbrt(S.v_new / S.v_old)};
     max_mol = std::max(max_mol, v.n_mol);
+    max_site = std::max(max_site, v.n_site);
+    S.in_volume = true;
   }
-  // energies at the current volume, then every box rescaled, its cells
-  // rebuilt for the new geometry, and the energies at the new volume
-  if ((rc = upload_items(b, K)) || (rc = launch_energy_pass(b, K, max_mol, 0))) return rc;
+  const hipStream_t st = b->vstream;
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * K,
+                               hipMemcpyHostToDevice, st));
   for (int k = 0; k < K; k++) {
     const int s = pend[k];
     BoxView& v = b->h_table[s].bv;
     for (int d = 0; d < 3; d++) v.L[d] = b->slots[s].L_old[d] * b->h_items[k].f;
     max_cells = std::max(max_cells, slot_set_geometry(b, s));
+    if ((rc = upload_slot(b, s, st))) return rc;
   }
-  if ((rc = upload_table(b))) return rc;
-  hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, b->stream,
+  hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  if ((rc = launch_cells_pass(b, K, max_mol, max_cells)) ||
-      (rc = launch_energy_pass(b, K, max_mol, 1)))
+  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st)) ||
+      (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st)))
     return rc;
   MCGPU_HIP_TRY(hipMemcpyAsync(b->h_vol, b->d_volout, sizeof(double) * 2 * K,
-                               hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  // decisions; rejected boxes are listed again for the undo
+                               hipMemcpyDeviceToHost, st));
+  MCGPU_HIP_TRY(hipEventRecord(b->ev_vol, st));
+  b->vol_list = pend;
+  b->vol_inflight = true;
+  return MCGPU_OK;
+}
+
+// Decides the volume pass in flight (waiting for it) and queues the undo of
+// rejected rescales and the uploads of the decided state on the set-up
+// stream; the boxes rejoin the next tick.
+int finish_volume_pass(mcgpu_batch* b) {
+  if (!b->vol_inflight) return MCGPU_OK;
+  int rc;
+  MCGPU_HIP_TRY(hipEventSynchronize(b->ev_vol));
+  b->vol_inflight = false;
+  const hipStream_t st = b->sstream;
+  MCGPU_HIP_TRY(hipStreamWaitEvent(st, b->ev_vol, 0));
+  const int K = (int)b->vol_list.size();
   int nrej = 0, rej_mol = 0, rej_cells = 0;
-  std::vector<double> e(2 * K);
-  std::memcpy(e.data(), b->h_vol, sizeof(double) * 2 * K);
-  std::vector<VolItem> items(b->h_items, b->h_items + K);
   for (int k = 0; k < K; k++) {
-    const int s = pend[k];
+    const int s = b->vol_list[k];
     BatchSlot& S = b->slots[s];
     BoxView& v = b->h_table[s].bv;
     BoxCtl& c = b->h_ctl[s];
-    const double e_old = e[2 * k], e_new = e[2 * k + 1];
     const double lrc_new = S.lrc * S.v_old / S.v_new;
-    const double de = (e_new - e_old) + (lrc_new - S.lrc);
+    const double e_new = b->h_vol[2 * k + 1] + lrc_new;
+    const double de = e_new - c.energy;
     double prob = v.n_mol * (std::log(S.v_new) - std::log(S.v_old));
     prob -= (de + S.params.pressure * (S.v_new - S.v_old)) / v.kT;
     prob = std::exp(prob);
     S.vol_moves++;
     if (prob >= S.u_accept) {
       S.vol_acc++;
-      c.energy += de;
+      c.energy = e_new;
       S.lrc = lrc_new;
     } else {
       for (int d = 0; d < 3; d++) v.L[d] = S.L_old[d];
       rej_cells = std::max(rej_cells, slot_set_geometry(b, s));
       rej_mol = std::max(rej_mol, v.n_mol);
-      b->h_items[nrej++] = items[k];
+      b->h_undo[nrej++] = VolItem{s, 0, 1.0};
+      if ((rc = upload_slot(b, s, st))) return rc;
     }
     c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
     c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
+    S.in_volume = false;
+    if ((rc = upload_ctl(b, s, st)) || (rc = upload_rng(b, s, st))) return rc;
   }
   if (nrej) {
-    if ((rc = upload_items(b, nrej)) || (rc = upload_table(b))) return rc;
-    hipLaunchKernelGGL(k_restore_multi, dim3((rej_mol + 255) / 256, nrej), dim3(256), 0,
-                       b->stream, b->d_table, b->d_items);
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_undo, b->h_undo, sizeof(VolItem) * nrej,
+                                 hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_restore_multi, dim3((rej_mol + 255) / 256, nrej), dim3(256), 0, st,
+                       b->d_table, b->d_undo);
     MCGPU_HIP_TRY(hipGetLastError());
-    if ((rc = launch_cells_pass(b, nrej, rej_mol, rej_cells))) return rc;
+    if ((rc = launch_cells_pass(b, b->d_undo, nrej, rej_mol, rej_cells, st))) return rc;
   }
-  // the host mirrors are current for every box between launches
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * b->max_boxes,
-                               hipMemcpyHostToDevice, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_rng, b->h_rng, sizeof(Rng31) * b->max_boxes,
-                               hipMemcpyHostToDevice, b->stream));
+  b->vol_list.clear();
   return MCGPU_OK;
 }
 
-// Consumes a finished launch: reads timing and control state, runs pending
-// volume moves.
-int batch_sync(mcgpu_batch* b) {
-  if (!b->pending && !b->stale_ctl) return MCGPU_OK;
-  if (b->stale_ctl && !b->pending) {
-    int rc = readback(b);
-    if (rc) return rc;
-  }
+// Tick boundary: consumes a finished tick (its read-back merged for the
+// boxes it ran) and runs the volume moves of the boxes that reached a
+// volume stop in it: one batched pass, then the decisions, so those boxes
+// run again in the next tick.  Without a tick in flight it does nothing.
+int batch_boundary(mcgpu_batch* b) {
+  if (!b->pending) return MCGPU_OK;
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-  b->stale_ctl = false;
-  const bool launched = b->pending;
   b->pending = false;
+  for (int s : b->launched) {
+    b->h_ctl[s] = b->h_ctl_rb[s];
+    b->h_rng[s] = b->h_rng_rb[s];
+    b->slots[s].dirty = false;
+  }
   if (b->timing_pending) {
     float ms = 0.f;
     MCGPU_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -963,12 +1149,37 @@ int batch_sync(mcgpu_batch* b) {
     b->kernel_launches++;
     b->timing_pending = false;
   }
-  if (!launched) return MCGPU_OK;
+  int rc = finish_volume_pass(b);
+  if (rc) return rc;
   std::vector<int> pend;
-  for (int s = 0; s < b->max_boxes; s++)
+  for (int s : b->launched)
     if (b->slots[s].used && b->h_ctl[s].status == kVolumePending) pend.push_back(s);
-  if (!pend.empty()) return batch_volume_moves(b, pend);
+  b->launched.clear();
+  if (pend.empty()) return MCGPU_OK;
+  if ((rc = start_volume_pass(b, pend))) return rc;
+  return finish_volume_pass(b);
+}
+
+// Control blocks of slots set up since their last tick, from the device.
+int refresh_dirty(mcgpu_batch* b, int only_slot) {
+  bool any = false;
+  for (int s = 0; s < b->max_boxes; s++)
+    if (b->slots[s].used && b->slots[s].dirty && (only_slot < 0 || s == only_slot)) {
+      if (!any) MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
+      any = true;
+      MCGPU_HIP_TRY(hipMemcpy(b->h_ctl + s, b->d_ctl + s, sizeof(BoxCtl),
+                              hipMemcpyDeviceToHost));
+      b->slots[s].dirty = false;
+    }
   return MCGPU_OK;
+}
+
+// Everything queued is done and every mirror is current.
+int batch_quiesce(mcgpu_batch* b) {
+  int rc;
+  if ((rc = batch_boundary(b)) || (rc = finish_volume_pass(b))) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
+  return refresh_dirty(b, -1);
 }
 
 void free_slot(mcgpu_batch* b, int s) {
@@ -1008,25 +1219,34 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
       khz > 0)
     b->clock_khz = khz;
   const size_t M = max_boxes;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&b->d_table, sizeof(BatchBox) * M) != hipSuccess ||
-      hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) != hipSuccess ||
-      hipMalloc(&b->d_rng, sizeof(Rng31) * M) != hipSuccess ||
-      hipMalloc(&b->d_active, sizeof(int32_t) * kGroups * (M + 1)) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * kGroups * (M + 1), 0) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_vol, sizeof(double) * 4 * M, 0) != hipSuccess ||
-      hipMalloc(&b->d_items, sizeof(VolItem) * M) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) != hipSuccess ||
-      hipMalloc(&b->d_volout, sizeof(double) * 2 * M) != hipSuccess) {
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_vol, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming) == hipSuccess;
+  for (int g = 0; ok && g < kGroups - 1; g++)
+    ok = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming) == hipSuccess;
+  b->vstream = b->sstream = b->stream;
+  ok = ok && hipMalloc(&b->d_table, sizeof(BatchBox) * M) == hipSuccess &&
+       hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) == hipSuccess &&
+       hipMalloc(&b->d_rng, sizeof(Rng31) * M) == hipSuccess &&
+       hipMalloc(&b->d_active, sizeof(int32_t) * kGroups * (M + 1)) == hipSuccess &&
+       hipMalloc(&b->d_items, sizeof(VolItem) * M) == hipSuccess &&
+       hipMalloc(&b->d_undo, sizeof(VolItem) * M) == hipSuccess &&
+       hipMalloc(&b->d_one, sizeof(VolItem) * M) == hipSuccess &&
+       hipMalloc(&b->d_volout, sizeof(double) * 2 * M) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_ctl_rb, sizeof(BoxCtl) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_rng_rb, sizeof(Rng31) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * kGroups * (M + 1), 0) ==
+           hipSuccess &&
+       hipHostMalloc((void**)&b->h_vol, sizeof(double) * 2 * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) == hipSuccess &&
+       hipHostMalloc((void**)&b->h_undo, sizeof(VolItem) * M, 0) == hipSuccess;
+  if (!ok) {
     set_error("mcgpu_batch_create: allocation failed");
     return fail(MCGPU_ERR_NOMEM);
   }
@@ -1034,7 +1254,14 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
   std::memset(b->h_ctl, 0, sizeof(BoxCtl) * M);
   std::memset(b->h_rng, 0, sizeof(Rng31) * M);
   for (size_t s = 0; s < M; s++) b->h_ctl[s].status = kDone;
-  if (hipMemcpy(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * M, hipMemcpyHostToDevice) != hipSuccess) {
+  {
+    std::vector<VolItem> one(M);
+    for (size_t s = 0; s < M; s++) one[s] = VolItem{(int32_t)s, 0, 1.0};
+    ok = hipMemcpy(b->d_one, one.data(), sizeof(VolItem) * M, hipMemcpyHostToDevice) ==
+         hipSuccess;
+  }
+  if (!ok ||
+      hipMemcpy(b->d_ctl, b->h_ctl, sizeof(BoxCtl) * M, hipMemcpyHostToDevice) != hipSuccess) {
     set_error("mcgpu_batch_create: upload failed");
     return fail(MCGPU_ERR_HIP);
   }
@@ -1047,23 +1274,18 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (hipStream_t s : b->side)
     if (s) (void)hipStreamSynchronize(s);
+
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->d_table) (void)hipFree(b->d_table);
-  if (b->d_ctl) (void)hipFree(b->d_ctl);
-  if (b->d_rng) (void)hipFree(b->d_rng);
-  if (b->d_active) (void)hipFree(b->d_active);
-  if (b->h_table) (void)hipHostFree(b->h_table);
-  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
-  if (b->h_rng) (void)hipHostFree(b->h_rng);
-  if (b->h_active) (void)hipHostFree(b->h_active);
-  if (b->h_vol) (void)hipHostFree(b->h_vol);
-  if (b->d_items) (void)hipFree(b->d_items);
-  if (b->h_items) (void)hipHostFree(b->h_items);
-  if (b->d_volout) (void)hipFree(b->d_volout);
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
-  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (void* p : {(void*)b->d_table, (void*)b->d_ctl, (void*)b->d_rng, (void*)b->d_active,
+                  (void*)b->d_items, (void*)b->d_undo, (void*)b->d_one, (void*)b->d_volout})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->h_table, (void*)b->h_ctl, (void*)b->h_ctl_rb, (void*)b->h_rng,
+                  (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_vol, (void*)b->h_items,
+                  (void*)b->h_undo})
+    if (p) (void)hipHostFree(p);
+  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_vol, b->ev_ready})
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1093,6 +1315,9 @@ extern "C" int mcgpu_box_footprint(const mcgpu_box_desc* d,
   return MCGPU_OK;
 }
 
+// Admission: the box's arrays are uploaded and its cells, start energy and
+// control block set up on the set-up stream, beside a tick in flight (a
+// tick only touches the boxes it runs); the box joins the next tick.
 extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
                                const mcgpu_run_params* p, int32_t* slot_out) {
   if (!b || !d || !p || !slot_out) return MCGPU_ERR_INVALID;
@@ -1109,8 +1334,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
     set_error("mcgpu_batch_add: bad run parameters");
     return MCGPU_ERR_INVALID;
   }
-  int rc = batch_sync(b);
-  if (rc) return rc;
+  int rc;
   int s = -1;
   for (int k = 0; k < b->max_boxes; k++)
     if (!b->slots[k].used) {
@@ -1196,6 +1420,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.tr_de = (double*)(base + L.trd);
   E.tr_acc = (int8_t*)(base + L.tra);
   rng31_seed(b->h_rng[s], p->seed);
+  const hipStream_t st = b->sstream;
   const int32_t zero[3] = {0, 0, 0};
   struct Copy {
     void* dst;
@@ -1218,40 +1443,46 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
                 {base + L.vcount, zero, 4},
                 {b->d_rng + s, b->h_rng + s, sizeof(Rng31)}};
   for (const Copy& c : copies) {
-    if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, b->stream) !=
-        hipSuccess) {
+    if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
       free_slot(b, s);
       set_error("mcgpu_batch_add: upload failed");
       return MCGPU_ERR_HIP;
     }
   }
   // The sources are the caller's (and this call's) host arrays: wait for
-  // the copies; the set-up kernels below stay asynchronous.
-  if (hipStreamSynchronize(b->stream) != hipSuccess) {
+  // the copies (the set-up stream only); the set-up kernels stay queued.
+  if (hipStreamSynchronize(st) != hipSuccess) {
     free_slot(b, s);
     set_error("mcgpu_batch_add: upload failed");
     return MCGPU_ERR_HIP;
   }
-  if ((rc = launch_refresh_primary(v, S.n_pidx, b->stream)) ||
-      (p->neighbor == MCGPU_NEIGHBOR_CELL && (rc = slot_rebuild_cells(b, s))) ||
-      (rc = slot_energy_async(b, s, 0))) {
+  int ncell = 0;
+  if (p->neighbor == MCGPU_NEIGHBOR_CELL) ncell = slot_set_geometry(b, s);
+  if ((rc = upload_slot(b, s, st)) || (rc = launch_refresh_primary(v, S.n_pidx, st)) ||
+      (ncell > 0 && (rc = launch_cells_build(v, st)))) {
     free_slot(b, s);
     return rc;
   }
+  // start energy: the full-energy pass over this one box
+  hipLaunchKernelGGL(k_energy_waves, dim3(std::min((v.n_mol + 3) / 4, 2048), 1), dim3(256), 0,
+                     st, b->d_table, b->d_one + s);
+  hipLaunchKernelGGL(k_reduce_multi, dim3(1), dim3(256), 0, st, b->d_table, b->d_one + s, 0,
+                     E.total + 2);
+  MCGPU_HIP_TRY(hipGetLastError());
   BoxCtl& c = b->h_ctl[s];
   c = BoxCtl{};
   c.steps_target = p->steps;
   c.next_stop = (p->ensemble == MCGPU_NPT) ? std::min<int64_t>(p->steps, p->volume_interval)
                                            : p->steps;
   c.status = p->steps > 0 ? kRunning : kDone;
-  if ((rc = upload_slot(b, s)) || (rc = upload_ctl(b, s))) {
+  if ((rc = upload_ctl(b, s, st))) {
     free_slot(b, s);
     return rc;
   }
-  hipLaunchKernelGGL(k_set_start_energy, dim3(1), dim3(64), 0, b->stream, b->d_ctl + s,
-                     (const double*)(base + L.total), p->energy_offset);
+  hipLaunchKernelGGL(k_set_start_energy, dim3(1), dim3(64), 0, st, b->d_ctl + s,
+                     (const double*)E.total, p->energy_offset);
   MCGPU_HIP_TRY(hipGetLastError());
-  b->stale_ctl = true;  // the device now holds the start energy
+  S.dirty = true;  // the device now holds the start energy
   *slot_out = s;
   return MCGPU_OK;
 }
@@ -1261,29 +1492,49 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
     set_error("mcgpu_batch_remove: bad slot");
     return MCGPU_ERR_RANGE;
   }
-  int rc = batch_sync(b);
+  int rc = batch_boundary(b);
   if (rc) return rc;
+  if (b->slots[s].in_volume && (rc = finish_volume_pass(b))) return rc;
+  if (b->slots[s].dirty) MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
   b->h_ctl[s] = BoxCtl{};
   b->h_ctl[s].status = kDone;
-  if ((rc = upload_ctl(b, s))) return rc;
+  if ((rc = upload_ctl(b, s, b->sstream))) return rc;
   free_slot(b, s);
   return MCGPU_OK;
 }
 
+// One tick: every running box (not in a volume move) advances up to
+// `sweeps` moves in one graph launch.  The previous tick is consumed first;
+// with sync the call returns after this tick is consumed too (the volume
+// pass it started then runs beside the next tick).
 extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   if (!b || sweeps < 1) return MCGPU_ERR_INVALID;
-  int rc = batch_sync(b);
+  int rc = batch_boundary(b);
   if (rc) return rc;
   const int stride = b->max_boxes + 1;
   int n[kGroups] = {};
-  for (int s = 0; s < b->max_boxes; s++)
-    if (b->slots[s].used && b->h_ctl[s].status == kRunning) {
-      const int g = b->slots[s].group;
-      b->h_active[g * stride + 1 + n[g]++] = s;
-    }
   int total = 0;
-  for (int g = 0; g < kGroups; g++) total += n[g];
+  for (int pass = 0; pass < 2 && total == 0; pass++) {
+    // nothing runnable while a volume pass is in flight: decide it first
+    if (pass == 1) {
+      if (!b->vol_inflight) break;
+      if ((rc = finish_volume_pass(b))) return rc;
+    }
+    for (int g = 0; g < kGroups; g++) n[g] = 0;
+    b->launched.clear();
+    for (int s = 0; s < b->max_boxes; s++) {
+      const BatchSlot& S = b->slots[s];
+      if (S.used && !S.in_volume && b->h_ctl[s].status == kRunning) {
+        b->h_active[S.group * stride + 1 + n[S.group]++] = s;
+        b->launched.push_back(s);
+      }
+    }
+    total = (int)b->launched.size();
+  }
   if (total == 0) return MCGPU_OK;
+  // set-up work queued so far (admissions, decided volume moves) first
+  MCGPU_HIP_TRY(hipEventRecord(b->ev_ready, b->sstream));
+  MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_ready, 0));
   for (int g = 0; g < kGroups; g++) {
     b->h_active[g * stride] = n[g];
     MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, b->h_active + g * stride,
@@ -1332,10 +1583,14 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
   MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
-  if ((rc = readback(b))) return rc;
+  // read-back of the boxes' control blocks and generators into staging
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl_rb, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
+                               hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng_rb, b->d_rng, sizeof(Rng31) * b->max_boxes,
+                               hipMemcpyDeviceToHost, b->stream));
   b->pending = true;
   b->timing_pending = true;
-  if (sync) return batch_sync(b);
+  if (sync) return batch_boundary(b);
   return MCGPU_OK;
 }
 
@@ -1344,8 +1599,8 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
     set_error("mcgpu_batch_stats: bad slot");
     return MCGPU_ERR_RANGE;
   }
-  int rc = batch_sync(b);
-  if (rc) return rc;
+  int rc = batch_boundary(b);
+  if (rc || (rc = refresh_dirty(b, s))) return rc;
   const BoxCtl& c = b->h_ctl[s];
   out->steps_done = c.steps_done;
   out->accepted = c.accepted;
@@ -1355,7 +1610,9 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   out->start_energy = c.start_energy;
   out->work_candidates = c.work_cand;
   out->work_sites = c.work_sites + c.work_self;
-  for (int k = 0; k < 3; k++) out->box[k] = b->h_table[s].bv.L[k];
+  // a box in a volume move reports its edges before the move until decided
+  for (int k = 0; k < 3; k++)
+    out->box[k] = b->slots[s].in_volume ? b->slots[s].L_old[k] : b->h_table[s].bv.L[k];
   out->done = c.status == kDone;
   out->fault = c.fault ? c.fault : b->slots[s].fault;
   return MCGPU_OK;
@@ -1367,7 +1624,7 @@ extern "C" int mcgpu_batch_download(mcgpu_batch* b, int32_t s, double* x,
     set_error("mcgpu_batch_download: bad argument");
     return MCGPU_ERR_RANGE;
   }
-  int rc = batch_sync(b);
+  int rc = batch_quiesce(b);
   if (rc) return rc;
   const BoxView& v = b->h_table[s].bv;
   const size_t nb = 8 * (size_t)v.n_site;
@@ -1385,7 +1642,7 @@ extern "C" int mcgpu_batch_trace(mcgpu_batch* b, int32_t s, int32_t* mol,
     set_error("mcgpu_batch_trace: bad argument");
     return MCGPU_ERR_RANGE;
   }
-  int rc = batch_sync(b);
+  int rc = batch_quiesce(b);
   if (rc) return rc;
   const BatchBox& E = b->h_table[s];
   if (!E.record) {
@@ -1406,7 +1663,7 @@ extern "C" int mcgpu_batch_trace(mcgpu_batch* b, int32_t s, int32_t* mol,
 extern "C" int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms,
                                      int64_t* launches) {
   if (!b || !avg_ms || !launches) return MCGPU_ERR_INVALID;
-  int rc = batch_sync(b);
+  int rc = batch_boundary(b);
   if (rc) return rc;
   *launches = b->kernel_launches;
   *avg_ms = b->kernel_launches ? b->kernel_ms_sum / b->kernel_launches : 0.0;

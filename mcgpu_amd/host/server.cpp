@@ -116,6 +116,8 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
   // admission must complete within this many seconds.
   const double kStallLimit = 60.0;
   double last_progress = 0.0;
+  double t_admit = 0.0, t_run = 0.0, t_retire = 0.0;  // host time per phase
+  double kernel_ms = 0.0;  // device time of the ticks (HIP events), summed
   while (completed < s->jobs.size()) {
     // admit every arrived job that fits (FIFO; the head waits for memory)
     const double t = now();
@@ -143,6 +145,7 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       last_progress = j.res.admit_s;
       resident.push_back((int32_t)order[next - 1]);
     }
+    t_admit += now() - t;
     peak = std::max<int64_t>(peak, (int64_t)resident.size());
     const bool arrived = next < order.size() && s->jobs[order[next]].res.arrival_s <= now();
     if ((arrived || !resident.empty()) && now() - last_progress > kStallLimit) {
@@ -163,6 +166,7 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
     const double tt = now();
     int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 1);
     if (rc != MCGPU_OK) return rc;
+    t_run += now() - tt;
     s->stats.ticks++;
     last_progress = now();
     if (log_every > 0 && now() - tt > log_every) {
@@ -171,13 +175,18 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       std::fflush(stderr);
     }
     if (log_every > 0 && now() >= next_log) {
-      std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu admitted=%zu done=%zu moves=%lld\n",
+      double kms = 0.0;
+      int64_t nl = 0;
+      if (mcgpu_batch_kernel_ms(s->batch, &kms, &nl) == MCGPU_OK) kernel_ms += kms * nl;
+      std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu admitted=%zu done=%zu moves=%lld "
+                   "host s: admit %.2f run %.2f (kernels %.2f) retire %.2f\n",
                    now(), (long long)s->stats.ticks, resident.size(), next, completed,
-                   (long long)s->stats.total_moves);
+                   (long long)s->stats.total_moves, t_admit, t_run, kernel_ms / 1e3, t_retire);
       std::fflush(stderr);
       next_log = now() + log_every;
     }
     // retire finished jobs
+    const double tr = now();
     for (size_t k = 0; k < resident.size();) {
       Job& j = s->jobs[resident[k]];
       rc = mcgpu_batch_stats(s->batch, j.slot, &st);
@@ -207,6 +216,7 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
         k++;
       }
     }
+    t_retire += now() - tr;
   }
   s->stats.wall_s = now();
   s->stats.peak_resident = peak;
@@ -215,7 +225,8 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
   double avg_ms = 0;
   int64_t launches = 0;
   if (mcgpu_batch_kernel_ms(s->batch, &avg_ms, &launches) == MCGPU_OK)
-    s->stats.kernel_ms_total = avg_ms * launches;
+    kernel_ms += avg_ms * launches;
+  s->stats.kernel_ms_total = kernel_ms;
   return MCGPU_OK;
 }
 
