@@ -236,7 +236,7 @@ def main() -> int:
     os.environ.setdefault("MCGPU_SERVER_LOG", "10")
     # warm-up: same server path, saturated (all warm-up jobs at t=0)
     wj = [dict(j, arrival=0.0, steps=min(j["steps"], 2000)) for j in warm]
-    serve(wj, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
+    _, wst = serve(wj, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
     print("# warm-up done", file=sys.stderr, flush=True)
 
     # timed region: exactly K jobs of the stream
@@ -315,6 +315,9 @@ def main() -> int:
             "kernel": "k_batch (one graph launch: 256-, 512- and 1024-lane instances on three streams)",
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": st.kernel_ms_total / max(st.ticks, 1),
+            # graph launches of the whole process (warm-up, timed and
+            # capacity serving), to normalise per-process counter totals
+            "graph_launches_in_run": int(wst.ticks + st.ticks + cst.ticks),
         },
         "cpu_baseline": cpu,
     }
