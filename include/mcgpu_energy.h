@@ -229,8 +229,18 @@ int mcgpu_batch_set_tick_budget(mcgpu_batch* b, double microseconds);
 /* Arena bytes in use, peak, and capacity (the admission budget). */
 int mcgpu_batch_memory(mcgpu_batch* b, size_t* used, size_t* peak,
                        size_t* capacity);
-/* Work-group size the batch chose for a box (256 or 512 lanes). */
+/* Work-group size the batch uses for a box (256, 512 or 1024 lanes). */
 int mcgpu_batch_group(mcgpu_batch* b, int32_t slot, int32_t* threads);
+/* Work-group-size tuner (on by default; active only with a tick budget):
+ * per box class (neighbour source, precision, largest molecule, molecule
+ * count in quarter-octaves) the first box of the class runs one full tick
+ * at each size and the size with the most moves per tick is kept for the
+ * class.  Off: sizes come from a density estimate of the per-move work. */
+int mcgpu_batch_set_autotune(mcgpu_batch* b, int32_t on);
+/* The tuned classes: key, chosen lanes, and moves per tick measured at
+ * 256 / 512 / 1024 lanes (3 per class); n = number of tuned classes. */
+int mcgpu_batch_tuning(mcgpu_batch* b, int64_t* keys, int32_t* threads,
+                       double* moves_per_tick, int32_t cap, int32_t* n);
 /* Average device duration (ms) of the batch kernel over the last
  * mcgpu_batch_run calls since the previous query, and their count. */
 int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms, int64_t* launches);

@@ -333,6 +333,20 @@ class Batch:
     def set_tick_budget(self, microseconds: float) -> None:
         check(lib().mcgpu_batch_set_tick_budget(self._b, C.c_double(microseconds)))
 
+    def set_autotune(self, on: bool) -> None:
+        check(lib().mcgpu_batch_set_autotune(self._b, int(bool(on))))
+
+    def tuning(self):
+        """Tuned box classes: [(key, lanes chosen, (moves/tick at 256, 512, 1024))]."""
+        n = C.c_int32()
+        check(lib().mcgpu_batch_tuning(self._b, None, None, None, 0, C.byref(n)))
+        k = n.value
+        keys = (C.c_int64 * max(k, 1))()
+        thr = (C.c_int32 * max(k, 1))()
+        mv = (C.c_double * max(3 * k, 1))()
+        check(lib().mcgpu_batch_tuning(self._b, keys, thr, mv, k, C.byref(n)))
+        return [(keys[i], thr[i], tuple(mv[3 * i:3 * i + 3])) for i in range(k)]
+
     def group_threads(self, slot: int) -> int:
         t = C.c_int32()
         check(lib().mcgpu_batch_group(self._b, int(slot), C.byref(t)))

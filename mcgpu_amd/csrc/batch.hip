@@ -888,6 +888,9 @@ struct BatchSlot {
   int64_t vol_moves = 0, vol_acc = 0;
   int fault = 0;
   bool in_volume = false;  // a volume move is in flight (not in ticks)
+  int64_t tune_key = 0;    // box class for the work-group-size tuner
+  int trial = -1;          // group being measured for the class (-1: none)
+  int64_t steps_at_launch = 0;
   bool dirty = false;      // the device's control block is newer than h_ctl
   // volume move in flight
   double v_old = 0.0, v_new = 0.0, u_accept = 0.0, L_old[3] = {0, 0, 0};
@@ -944,6 +947,15 @@ struct mcgpu_batch {
   hipGraphExec_t gexec = nullptr;
   int graph_sweeps = -1;
   uint64_t graph_version = 0, version = 1;
+  // work-group-size tuner: per box class, moves per full tick at each size
+  struct Tune {
+    int best = -1;
+    double moves[kGroups] = {};
+    int samples[kGroups] = {};
+    bool trialing = false;
+  };
+  std::map<int64_t, Tune> tune;
+  bool autotune = true;
   // timing
   double kernel_ms_sum = 0.0;
   int64_t kernel_launches = 0;
@@ -1129,6 +1141,48 @@ int finish_volume_pass(mcgpu_batch* b) {
   return MCGPU_OK;
 }
 
+// Work-group-size tuner.  Per box class (neighbour source, pair precision,
+// largest molecule, molecule count in quarter-octaves) the first box of the
+// class runs one tick at each work-group size; all boxes of a tick get the
+// same time budget, so the moves it made in a full tick measure its speed
+// at that size.  The fastest size is kept for the class; until then other
+// boxes of the class use the density estimate (choose_group).
+int64_t tune_key_of(const mcgpu_box_desc& d, const mcgpu_run_params& p, int max_len) {
+  const int64_t nb = (int64_t)std::lround(4.0 * std::log2((double)std::max(d.n_molecules, 1)));
+  return ((int64_t)p.neighbor * 4 + p.precision) * 1000000 + (int64_t)max_len * 1000 + nb;
+}
+
+void tune_sample(mcgpu_batch* b, int s) {
+  BatchSlot& S = b->slots[s];
+  if (S.trial < 0) return;
+  const BoxCtl& c = b->h_ctl[s];
+  const int64_t moved = c.steps_done - S.steps_at_launch;
+  // only full ticks measure speed: the box ran until the time budget
+  if (c.status != kRunning || moved < 1 || c.steps_done >= c.next_stop) return;
+  auto& T = b->tune[S.tune_key];
+  T.moves[S.trial] += (double)moved;
+  T.samples[S.trial]++;
+  int next = -1;
+  for (int g = 0; g < kGroups; g++)
+    if (T.samples[g] == 0) {
+      next = g;
+      break;
+    }
+  if (next >= 0) {
+    S.trial = next;
+    S.group = next;
+    return;
+  }
+  int best = 0;
+  for (int g = 1; g < kGroups; g++)
+    if (T.moves[g] / T.samples[g] > T.moves[best] / T.samples[best]) best = g;
+  T.best = best;
+  T.trialing = false;
+  S.trial = -1;
+  for (int k = 0; k < b->max_boxes; k++)
+    if (b->slots[k].used && b->slots[k].tune_key == S.tune_key) b->slots[k].group = best;
+}
+
 // Tick boundary: consumes a finished tick (its read-back merged for the
 // boxes it ran) and runs the volume moves of the boxes that reached a
 // volume stop in it: one batched pass, then the decisions, so those boxes
@@ -1141,6 +1195,7 @@ int batch_boundary(mcgpu_batch* b) {
     b->h_ctl[s] = b->h_ctl_rb[s];
     b->h_rng[s] = b->h_rng_rb[s];
     b->slots[s].dirty = false;
+    if (b->budget_ticks > 0) tune_sample(b, s);
   }
   if (b->timing_pending) {
     float ms = 0.f;
@@ -1184,6 +1239,10 @@ int batch_quiesce(mcgpu_batch* b) {
 
 void free_slot(mcgpu_batch* b, int s) {
   if (!b->slots[s].used) return;
+  if (b->slots[s].trial >= 0) {
+    auto it = b->tune.find(b->slots[s].tune_key);
+    if (it != b->tune.end()) it->second.trialing = false;
+  }
   b->arena.release(b->slots[s].off);
   b->slots[s] = BatchSlot();
 }
@@ -1367,6 +1426,20 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   S.lrc = p->energy_offset;
   S.spread = pt.spread;
   S.group = choose_group(*d, p->neighbor, pt.max_len);
+  S.tune_key = tune_key_of(*d, *p, pt.max_len);
+  if (b->autotune && b->budget_ticks > 0) {
+    auto& T = b->tune[S.tune_key];
+    if (T.best >= 0) {
+      S.group = T.best;
+    } else if (!T.trialing) {  // this box measures the class
+      T.trialing = true;
+      for (int g = 0; g < kGroups; g++)
+        if (T.samples[g] == 0) {
+          S.trial = S.group = g;
+          break;
+        }
+    }
+  }
   char* base = b->arena.base() + off;
   BatchBox& E = b->h_table[s];
   E = BatchBox{};
@@ -1527,6 +1600,7 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
       if (S.used && !S.in_volume && b->h_ctl[s].status == kRunning) {
         b->h_active[S.group * stride + 1 + n[S.group]++] = s;
         b->launched.push_back(s);
+        b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
       }
     }
     total = (int)b->launched.size();
@@ -1678,6 +1752,31 @@ extern "C" int mcgpu_batch_memory(mcgpu_batch* b, size_t* used, size_t* peak,
   if (used) *used = b->arena.used();
   if (peak) *peak = b->arena.peak();
   if (capacity) *capacity = b->arena.capacity();
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_set_autotune(mcgpu_batch* b, int32_t on) {
+  if (!b) return MCGPU_ERR_INVALID;
+  b->autotune = on != 0;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_tuning(mcgpu_batch* b, int64_t* keys, int32_t* threads,
+                                  double* moves_per_tick, int32_t cap, int32_t* n) {
+  if (!b || !n) return MCGPU_ERR_INVALID;
+  int k = 0;
+  for (const auto& kv : b->tune) {
+    if (kv.second.best < 0) continue;
+    if (k < cap) {
+      if (keys) keys[k] = kv.first;
+      if (threads) threads[k] = kGroupNT[kv.second.best];
+      for (int g = 0; moves_per_tick && g < kGroups; g++)
+        moves_per_tick[kGroups * k + g] =
+            kv.second.samples[g] ? kv.second.moves[g] / kv.second.samples[g] : 0.0;
+    }
+    k++;
+  }
+  *n = k;
   return MCGPU_OK;
 }
 

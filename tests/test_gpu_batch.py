@@ -162,3 +162,31 @@ def test_batch_cells_equal_brute_over_boundary_crossings():
     assert np.array_equal(tc.mol, tb.mol)
     assert np.array_equal(tc.accepted, tb.accepted)
     assert de_ok(tc.de, tb.de)
+
+
+def test_batch_autotuner_picks_a_size_and_keeps_results():
+    """With a tick budget, the first box of a class runs one full tick at each
+    work-group size and the class keeps the size with the most moves per tick.
+    The box's moves and decisions do not depend on the sizes it ran at: its
+    trace equals a run at the density-estimate size without tuning."""
+    hb = m.HostBox.generate(m.WATER, 1000, density=0.997, cutoff=9.0, seed=5, mc_seed=3)
+    steps = 3000
+    b = m.Batch(4)
+    b.set_tick_budget(300.0)
+    s = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+    run_all(b, [s], steps, 2000)
+    tuned = b.tuning()
+    assert len(tuned) == 1
+    _, lanes, per_tick = tuned[0]
+    print(f"tuned: {lanes} lanes, moves per tick at 256/512/1024: {per_tick}")
+    assert all(x > 0 for x in per_tick)
+    assert lanes == (256, 512, 1024)[int(np.argmax(per_tick))]
+    assert b.group_threads(s) == lanes
+    ref = m.Batch(2)
+    ref.set_autotune(False)
+    r = ref.add(hb, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+    run_all(ref, [r], steps, 2000)
+    t1, t0 = b.trace(s), ref.trace(r)
+    assert np.array_equal(t1.mol, t0.mol)
+    assert np.array_equal(t1.accepted, t0.accepted)
+    assert de_ok(t1.de, t0.de)
