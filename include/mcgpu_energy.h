@@ -241,6 +241,10 @@ int mcgpu_batch_set_autotune(mcgpu_batch* b, int32_t on);
  * 256 / 512 / 1024 lanes (3 per class); n = number of tuned classes. */
 int mcgpu_batch_tuning(mcgpu_batch* b, int64_t* keys, int32_t* threads,
                        double* moves_per_tick, int32_t cap, int32_t* n);
+/* Host seconds spent so far waiting for ticks to finish, in NPT volume
+ * moves (their batched passes included), and launching ticks. */
+int mcgpu_batch_host_seconds(mcgpu_batch* b, double* wait, double* volume,
+                             double* launch);
 /* Average device duration (ms) of the batch kernel over the last
  * mcgpu_batch_run calls since the previous query, and their count. */
 int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms, int64_t* launches);

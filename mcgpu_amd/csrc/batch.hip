@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -40,8 +41,15 @@ struct BoxCtl {
   int64_t work_cand;   // candidate molecules examined (sum over moves)
   int64_t work_sites;  // sites of in-range molecules loaded
   int64_t work_self;   // sites of the moving molecule (old + proposed)
+  int64_t vol_moves;   // NPT volume moves tried / accepted
+  int64_t vol_acc;
   double energy;
   double start_energy;
+  double L[3];         // current edges
+  double lrc;          // current tail correction (scaled with 1/V)
+  // the volume move in flight
+  double v_old, v_new, u_accept;
+  double L_old[3];
   int32_t status;  // kRunning / kDone / kVolumePending
   int32_t fault;   // MCGPU_FAULT_* that stopped the box (0 = none)
 };
@@ -62,13 +70,22 @@ struct BatchBox {
   double max_r;
   int32_t precision;
   int32_t record;
+  // NPT: pressure (kcal/mol/A^3), relative volume step, moves between
+  // volume moves; cell geometry inputs (cutoff, primary spread, the cells
+  // the allocation holds; 0 = no cell lists)
+  double pressure;
+  double max_dv;
+  double cutoff;
+  double spread;
+  int32_t vol_interval;
+  int32_t ncell_cap;
 };
 
 // One box of a batched full-energy / rescale / cell pass.
 struct VolItem {
   int32_t slot;
-  int32_t pad;
-  double f;  // edge scale factor of the rescale
+  int32_t undo;  // rejected volume move: restore the coordinates
+  double f;      // edge scale factor of the rescale
 };
 
 // ------------------------------------------------------------- kernel
@@ -417,6 +434,91 @@ __global__ void k_restore_coords(double* x, double* y, double* z,
 // One launch covers every box of a list (blockIdx.y = list entry), so the
 // volume moves of all boxes due in a tick cost a handful of launches.
 
+// Cell geometry of a box for its current edges (the host's cell_geometry):
+// cells of width >= rc + 2 spread, cell mode while the grid fits the
+// allocation, else the all-molecule scan.
+__device__ void set_geometry_dev(BatchBox& B) {
+  BoxView& v = B.bv;
+  if (B.ncell_cap == 0) {
+    v.mode = MCGPU_NB_BRUTE;
+    return;
+  }
+  const double w = B.cutoff + 2.0 * B.spread + 1e-9;
+  int n[3];
+  for (int d = 0; d < 3; d++) {
+    n[d] = (int)floor(v.L[d] / w);
+    n[d] = n[d] < 1 ? 1 : n[d];
+  }
+  if (n[0] * n[1] * n[2] > B.ncell_cap) {
+    v.mode = MCGPU_NB_BRUTE;
+    return;
+  }
+  for (int d = 0; d < 3; d++) {
+    v.cells.n[d] = n[d];
+    v.cells.inv_w[d] = n[d] / v.L[d];
+  }
+  v.mode = MCGPU_NB_CELL;
+}
+
+// NPT volume move, first half, one thread per listed box (semantics: see
+// enqueue_volume_moves): u1 and u2 drawn from the box's own generator, the
+// new edges and cell geometry set, the scale factor handed to the rescale.
+__global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const int s = items[i].slot;
+  BatchBox& B = table[s];
+  BoxCtl& c = ctl[s];
+  Rng31 g = *B.rng;
+  const double u1 = random_real(rng31_next(g), 0.0, 1.0);
+  c.u_accept = random_real(rng31_next(g), 0.0, 1.0);
+  *B.rng = g;
+  BoxView& v = B.bv;
+  for (int d = 0; d < 3; d++) c.L_old[d] = v.L[d];
+  c.v_old = v.L[0] * v.L[1] * v.L[2];
+  c.v_new = c.v_old * (1.0 + B.max_dv * (2.0 * u1 - 1.0));
+  const double f = cbrt(c.v_new / c.v_old);
+  items[i].f = f;
+  items[i].undo = 0;
+  for (int d = 0; d < 3; d++) v.L[d] = c.L_old[d] * f;
+  set_geometry_dev(B);
+}
+
+// Second half, after the full energy at the new volume (out[2 i + 1]):
+// the acceptance test, the running energy and LRC on acceptance, the old
+// edges and geometry (and the undo flag for the coordinates) on rejection,
+// and the box's next volume stop.
+__global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
+                             const double* out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const int s = items[i].slot;
+  BatchBox& B = table[s];
+  BoxCtl& c = ctl[s];
+  const double lrc_new = c.lrc * c.v_old / c.v_new;
+  const double e_new = out[2 * i + 1] + lrc_new;
+  const double de = e_new - c.energy;
+  double prob = B.bv.n_mol * (log(c.v_new) - log(c.v_old));
+  prob -= (de + B.pressure * (c.v_new - c.v_old)) / B.bv.kT;
+  prob = exp_fp64(prob);
+  c.vol_moves++;
+  if (prob >= c.u_accept) {
+    c.vol_acc++;
+    c.energy = e_new;
+    c.lrc = lrc_new;
+    for (int d = 0; d < 3; d++) c.L[d] = B.bv.L[d];
+  } else {
+    for (int d = 0; d < 3; d++) B.bv.L[d] = c.L_old[d];
+    set_geometry_dev(B);
+    items[i].undo = 1;
+  }
+  c.next_stop = c.steps_target < c.steps_done + B.vol_interval ? c.steps_target
+                                                               : c.steps_done + B.vol_interval;
+  c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
+}
+
 // Full-system energy (SimulationStep.cpp:40-71: the sum over i of E_i(i),
 // partners j > i) of every listed box, one wave per molecule i: the lanes
 // scan the candidates (the cells around i's first primary, or all j > i),
@@ -668,6 +770,7 @@ __global__ __launch_bounds__(256) void k_scale_multi(const BatchBox* table,
 // A rejected rescale undone: molecule m's sites and primaries restored.
 __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
                                                        const VolItem* items) {
+  if (!items[blockIdx.y].undo) return;
   const BatchBox& B = table[items[blockIdx.y].slot];
   const BoxView& bv = B.bv;
   const int m = blockIdx.x * 256 + threadIdx.x;
@@ -883,37 +986,28 @@ struct BatchSlot {
   mcgpu_run_params params{};
   int32_t n_pidx = 0;
   int32_t group = 0;
-  double lrc = 0.0;    // current tail correction (scaled with 1/V under NPT)
   double spread = 0.0; // largest primary-to-first-primary distance
-  int64_t vol_moves = 0, vol_acc = 0;
   int fault = 0;
-  bool in_volume = false;  // a volume move is in flight (not in ticks)
   int64_t tune_key = 0;    // box class for the work-group-size tuner
   int trial = -1;          // group being measured for the class (-1: none)
   int64_t steps_at_launch = 0;
   bool dirty = false;      // the device's control block is newer than h_ctl
-  // volume move in flight
-  double v_old = 0.0, v_new = 0.0, u_accept = 0.0, L_old[3] = {0, 0, 0};
 };
 
 // Host side of a batch.  Streams: `stream` carries the ticks (group 0 of the
-// graph), the read-back after them and, between ticks, the set-up work
-// (admissions, NPT volume passes and their undo); `side[]` the other
-// work-group-size groups of a tick.  vstream and sstream name the streams
-// of volume passes and set-up work; both are `stream`: work beside a tick
-// delays the start of the tick's persistent work-groups by more than it
-// saves (measured), so it runs between ticks.
-// Ownership: a box's control block and generator state belong to the device
-// from its launch in a tick until that tick's read-back is merged into the
-// host mirrors; a box in a volume pass or being set up belongs to the host.
+// graph), the read-back after them and, between ticks, the NPT volume moves
+// (decided on the device, so the host never waits for them); `side[]` the
+// other work-group-size groups of a tick; `sstream` admissions (their
+// uploads, cells and start energy), which every tick waits for (ev_ready).
+// Ownership: control blocks and generator states are the device's; the host
+// mirrors them from each tick's read-back (merged for the boxes the tick
+// ran) and writes a box's only when it sets the box up.
 struct mcgpu_batch {
   hipStream_t stream = nullptr;
   hipStream_t side[kGroups - 1] = {};
-  hipStream_t vstream = nullptr;
   hipStream_t sstream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipEvent_t join[kGroups - 1] = {};
-  hipEvent_t ev_vol = nullptr;    // a volume pass's energies are on the host
   hipEvent_t ev_ready = nullptr;  // set-up work queued so far is done
   Arena arena;
   int32_t max_boxes = 0;
@@ -928,19 +1022,14 @@ struct mcgpu_batch {
   Rng31* h_rng_rb = nullptr;    // pinned read-back of a tick
   int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
   int32_t* h_active = nullptr;  // pinned     [count, slots...]
-  double* h_vol = nullptr;      // pinned [2 * max_boxes] volume-pass energies
   VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
-  VolItem* d_undo = nullptr;    // [max_boxes] rejected rescales to undo
-  VolItem* h_undo = nullptr;    // pinned
   VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
   double* d_volout = nullptr;   // [2 * max_boxes] energies of a volume pass
   long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
   double clock_khz = 100000.0;
   bool pending = false;         // a tick in flight
-  bool vol_inflight = false;    // a volume pass in flight
   std::vector<int> launched;    // slots of the tick in flight
-  std::vector<int> vol_list;    // slots of the volume pass in flight
   int n_active[kGroups] = {};
   // graph cache
   hipGraph_t graph = nullptr;
@@ -956,6 +1045,8 @@ struct mcgpu_batch {
   };
   std::map<int64_t, Tune> tune;
   bool autotune = true;
+  // host seconds: waiting for ticks, volume moves, launching ticks
+  double t_wait = 0.0, t_volume = 0.0, t_launch = 0.0;
   // timing
   double kernel_ms_sum = 0.0;
   int64_t kernel_launches = 0;
@@ -963,6 +1054,11 @@ struct mcgpu_batch {
 };
 
 namespace {
+
+double host_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 
 int upload_slot(mcgpu_batch* b, int s, hipStream_t st) {
   MCGPU_HIP_TRY(hipMemcpyAsync(b->d_table + s, b->h_table + s, sizeof(BatchBox),
@@ -996,7 +1092,7 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
   }
   int n[3];
   double iw[3];
-  cell_geometry(v.L, std::sqrt(v.cutoff2), S.spread, n, iw);
+  cell_geometry(v.L, b->h_table[s].cutoff, S.spread, n, iw);
   if (n[0] * n[1] * n[2] > S.lay.ncell) {
     v.mode = MCGPU_NB_BRUTE;
     return 0;
@@ -1034,111 +1130,49 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
   return MCGPU_OK;
 }
 
-// NPT volume moves of the boxes in `pend`, batched: every box rescaled, its
-// cells rebuilt and its full energy at the new volume computed, all queued
-// on the volume stream (the boxes sit out the ticks until the decision).
-// Semantics (the build's own; the reference's NPT path, Ensemble.cpp:9-28 /
-// SimulationStep.cpp:476-518, always grows the box by 1.1 and never
-// recomputes the energy):
+// NPT volume moves of the boxes in `pend`, batched and decided on the
+// device, queued between two ticks: u1, u2 from each box's generator, the
+// rescale (with backup), cells for the new geometry, the full energy, the
+// decision, and the undo of rejected moves.  Semantics (the build's own;
+// the reference's NPT path, Ensemble.cpp:9-28 / SimulationStep.cpp:476-518,
+// always grows the box by 1.1 and never recomputes the energy):
 //   V' = V (1 + delta (2 u1 - 1)); molecules translated so first primaries
 //   scale by (V'/V)^(1/3); E' = E_pairs(V') + LRC(V'); dE = E' - E, with E
 //   the box's running energy (start energy plus accepted move dE);
 //   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2, and then the
 //   running energy becomes E' exactly.
-// u1 and u2 are the box's next two draws of its own RNG stream.
-int start_volume_pass(mcgpu_batch* b, const std::vector<int>& pend) {
+int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
   const int K = (int)pend.size();
-  int max_mol = 0, max_site = 0, max_cells = 0;
+  int max_mol = 0, max_cells = 0;
   for (int k = 0; k < K; k++) {
     const int s = pend[k];
-    BatchSlot& S = b->slots[s];
-    BoxView& v = b->h_table[s].bv;
-    Rng31& g = b->h_rng[s];
-    const double u1 = random_real(rng31_next(g), 0.0, 1.0);
-    S.u_accept = random_real(rng31_next(g), 0.0, 1.0);
-    for (int d = 0; d < 3; d++) S.L_old[d] = v.L[d];
-    S.v_old = v.L[0] * v.L[1] * v.L[2];
-    S.v_new = S.v_old * (1.0 + S.params.max_volume_delta * (2.0 * u1 - 1.0));
-    b->h_items[k] = VolItem{s, 0, std::cbrt(S.v_new / S.v_old)};
-    max_mol = std::max(max_mol, v.n_mol);
-    max_site = std::max(max_site, v.n_site);
-    S.in_volume = true;
+    b->h_items[k] = VolItem{s, 0, 1.0};
+    max_mol = std::max(max_mol, b->h_table[s].bv.n_mol);
+    max_cells = std::max(max_cells, b->slots[s].lay.ncell);
+    // the decision puts the box back to running (it is below its target)
+    b->h_ctl[s].status = kRunning;
   }
-  const hipStream_t st = b->vstream;
+  const hipStream_t st = b->stream;
   MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * K,
                                hipMemcpyHostToDevice, st));
-  for (int k = 0; k < K; k++) {
-    const int s = pend[k];
-    BoxView& v = b->h_table[s].bv;
-    for (int d = 0; d < 3; d++) v.L[d] = b->slots[s].L_old[d] * b->h_items[k].f;
-    max_cells = std::max(max_cells, slot_set_geometry(b, s));
-    if ((rc = upload_slot(b, s, st))) return rc;
-  }
+  const int tb = (K + 63) / 64;
+  hipLaunchKernelGGL(k_vol_begin, dim3(tb), dim3(64), 0, st, b->d_table, b->d_ctl, b->d_items,
+                     K);
   hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
   if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st)) ||
       (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st)))
     return rc;
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_vol, b->d_volout, sizeof(double) * 2 * K,
-                               hipMemcpyDeviceToHost, st));
-  MCGPU_HIP_TRY(hipEventRecord(b->ev_vol, st));
-  b->vol_list = pend;
-  b->vol_inflight = true;
-  return MCGPU_OK;
-}
-
-// Decides the volume pass in flight (waiting for it) and queues the undo of
-// rejected rescales and the uploads of the decided state on the set-up
-// stream; the boxes rejoin the next tick.
-int finish_volume_pass(mcgpu_batch* b) {
-  if (!b->vol_inflight) return MCGPU_OK;
-  int rc;
-  MCGPU_HIP_TRY(hipEventSynchronize(b->ev_vol));
-  b->vol_inflight = false;
-  const hipStream_t st = b->sstream;
-  MCGPU_HIP_TRY(hipStreamWaitEvent(st, b->ev_vol, 0));
-  const int K = (int)b->vol_list.size();
-  int nrej = 0, rej_mol = 0, rej_cells = 0;
-  for (int k = 0; k < K; k++) {
-    const int s = b->vol_list[k];
-    BatchSlot& S = b->slots[s];
-    BoxView& v = b->h_table[s].bv;
-    BoxCtl& c = b->h_ctl[s];
-    const double lrc_new = S.lrc * S.v_old / S.v_new;
-    const double e_new = b->h_vol[2 * k + 1] + lrc_new;
-    const double de = e_new - c.energy;
-    double prob = v.n_mol * (std::log(S.v_new) - std::log(S.v_old));
-    prob -= (de + S.params.pressure * (S.v_new - S.v_old)) / v.kT;
-    prob = std::exp(prob);
-    S.vol_moves++;
-    if (prob >= S.u_accept) {
-      S.vol_acc++;
-      c.energy = e_new;
-      S.lrc = lrc_new;
-    } else {
-      for (int d = 0; d < 3; d++) v.L[d] = S.L_old[d];
-      rej_cells = std::max(rej_cells, slot_set_geometry(b, s));
-      rej_mol = std::max(rej_mol, v.n_mol);
-      b->h_undo[nrej++] = VolItem{s, 0, 1.0};
-      if ((rc = upload_slot(b, s, st))) return rc;
-    }
-    c.next_stop = std::min(c.steps_target, c.steps_done + S.params.volume_interval);
-    c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
-    S.in_volume = false;
-    if ((rc = upload_ctl(b, s, st)) || (rc = upload_rng(b, s, st))) return rc;
-  }
-  if (nrej) {
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_undo, b->h_undo, sizeof(VolItem) * nrej,
-                                 hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_restore_multi, dim3((rej_mol + 255) / 256, nrej), dim3(256), 0, st,
-                       b->d_table, b->d_undo);
-    MCGPU_HIP_TRY(hipGetLastError());
-    if ((rc = launch_cells_pass(b, b->d_undo, nrej, rej_mol, rej_cells, st))) return rc;
-  }
-  b->vol_list.clear();
-  return MCGPU_OK;
+  hipLaunchKernelGGL(k_vol_decide, dim3(tb), dim3(64), 0, st, b->d_table, b->d_ctl,
+                     b->d_items, K, (const double*)b->d_volout);
+  hipLaunchKernelGGL(k_restore_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
+                     b->d_table, b->d_items);
+  MCGPU_HIP_TRY(hipGetLastError());
+  // cells again for the restored boxes (a rebuild of an accepted box's
+  // lists reproduces them: same positions and geometry, ordered slots)
+  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st);
 }
 
 // Work-group-size tuner.  Per box class (neighbour source, pair precision,
@@ -1184,12 +1218,14 @@ void tune_sample(mcgpu_batch* b, int s) {
 }
 
 // Tick boundary: consumes a finished tick (its read-back merged for the
-// boxes it ran) and runs the volume moves of the boxes that reached a
-// volume stop in it: one batched pass, then the decisions, so those boxes
-// run again in the next tick.  Without a tick in flight it does nothing.
+// boxes it ran) and queues the volume moves of the boxes that reached a
+// volume stop in it, ahead of the next tick, in which those boxes run
+// again.  Without a tick in flight it does nothing.
 int batch_boundary(mcgpu_batch* b) {
   if (!b->pending) return MCGPU_OK;
+  const double t0 = host_now();
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->t_wait += host_now() - t0;
   b->pending = false;
   for (int s : b->launched) {
     b->h_ctl[s] = b->h_ctl_rb[s];
@@ -1204,15 +1240,15 @@ int batch_boundary(mcgpu_batch* b) {
     b->kernel_launches++;
     b->timing_pending = false;
   }
-  int rc = finish_volume_pass(b);
-  if (rc) return rc;
   std::vector<int> pend;
   for (int s : b->launched)
     if (b->slots[s].used && b->h_ctl[s].status == kVolumePending) pend.push_back(s);
   b->launched.clear();
   if (pend.empty()) return MCGPU_OK;
-  if ((rc = start_volume_pass(b, pend))) return rc;
-  return finish_volume_pass(b);
+  const double t1 = host_now();
+  const int rc = enqueue_volume_moves(b, pend);
+  b->t_volume += host_now() - t1;
+  return rc;
 }
 
 // Control blocks of slots set up since their last tick, from the device.
@@ -1232,9 +1268,16 @@ int refresh_dirty(mcgpu_batch* b, int only_slot) {
 // Everything queued is done and every mirror is current.
 int batch_quiesce(mcgpu_batch* b) {
   int rc;
-  if ((rc = batch_boundary(b)) || (rc = finish_volume_pass(b))) return rc;
+  if ((rc = batch_boundary(b))) return rc;
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
-  return refresh_dirty(b, -1);
+  // nothing in flight: the mirrors take the device's state for every box
+  MCGPU_HIP_TRY(hipMemcpy(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
+                          hipMemcpyDeviceToHost));
+  MCGPU_HIP_TRY(hipMemcpy(b->h_rng, b->d_rng, sizeof(Rng31) * b->max_boxes,
+                          hipMemcpyDeviceToHost));
+  for (BatchSlot& S : b->slots) S.dirty = false;
+  return MCGPU_OK;
 }
 
 void free_slot(mcgpu_batch* b, int s) {
@@ -1279,20 +1322,18 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
     b->clock_khz = khz;
   const size_t M = max_boxes;
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->sstream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
             hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->ev_vol, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; ok && g < kGroups - 1; g++)
     ok = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming) == hipSuccess;
-  b->vstream = b->sstream = b->stream;
   ok = ok && hipMalloc(&b->d_table, sizeof(BatchBox) * M) == hipSuccess &&
        hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) == hipSuccess &&
        hipMalloc(&b->d_rng, sizeof(Rng31) * M) == hipSuccess &&
        hipMalloc(&b->d_active, sizeof(int32_t) * kGroups * (M + 1)) == hipSuccess &&
        hipMalloc(&b->d_items, sizeof(VolItem) * M) == hipSuccess &&
-       hipMalloc(&b->d_undo, sizeof(VolItem) * M) == hipSuccess &&
        hipMalloc(&b->d_one, sizeof(VolItem) * M) == hipSuccess &&
        hipMalloc(&b->d_volout, sizeof(double) * 2 * M) == hipSuccess &&
        hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) == hipSuccess &&
@@ -1302,9 +1343,7 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
        hipHostMalloc((void**)&b->h_rng_rb, sizeof(Rng31) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * kGroups * (M + 1), 0) ==
            hipSuccess &&
-       hipHostMalloc((void**)&b->h_vol, sizeof(double) * 2 * M, 0) == hipSuccess &&
-       hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) == hipSuccess &&
-       hipHostMalloc((void**)&b->h_undo, sizeof(VolItem) * M, 0) == hipSuccess;
+       hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) == hipSuccess;
   if (!ok) {
     set_error("mcgpu_batch_create: allocation failed");
     return fail(MCGPU_ERR_NOMEM);
@@ -1336,18 +1375,19 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
 
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->sstream) (void)hipStreamSynchronize(b->sstream);
   for (void* p : {(void*)b->d_table, (void*)b->d_ctl, (void*)b->d_rng, (void*)b->d_active,
-                  (void*)b->d_items, (void*)b->d_undo, (void*)b->d_one, (void*)b->d_volout})
+                  (void*)b->d_items, (void*)b->d_one, (void*)b->d_volout})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->h_table, (void*)b->h_ctl, (void*)b->h_ctl_rb, (void*)b->h_rng,
-                  (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_vol, (void*)b->h_items,
-                  (void*)b->h_undo})
+                  (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_items})
     if (p) (void)hipHostFree(p);
-  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_vol, b->ev_ready})
+  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_ready})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->sstream) (void)hipStreamDestroy(b->sstream);
   for (hipStream_t s : b->side)
     if (s) (void)hipStreamDestroy(s);
   delete b;
@@ -1423,7 +1463,6 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   S.lay = L;
   S.params = *p;
   S.n_pidx = d->n_primary;
-  S.lrc = p->energy_offset;
   S.spread = pt.spread;
   S.group = choose_group(*d, p->neighbor, pt.max_len);
   S.tune_key = tune_key_of(*d, *p, pt.max_len);
@@ -1489,6 +1528,12 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.bx = (double*)(base + L.bx);
   E.by = (double*)(base + L.by);
   E.bz = (double*)(base + L.bz);
+  E.pressure = p->pressure;
+  E.max_dv = p->max_volume_delta;
+  E.cutoff = d->cutoff;
+  E.spread = pt.spread;
+  E.vol_interval = p->volume_interval;
+  E.ncell_cap = L.ncell;
   E.tr_mol = (int32_t*)(base + L.trm);
   E.tr_de = (double*)(base + L.trd);
   E.tr_acc = (int8_t*)(base + L.tra);
@@ -1544,6 +1589,8 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   MCGPU_HIP_TRY(hipGetLastError());
   BoxCtl& c = b->h_ctl[s];
   c = BoxCtl{};
+  for (int k = 0; k < 3; k++) c.L[k] = d->box[k];
+  c.lrc = p->energy_offset;
   c.steps_target = p->steps;
   c.next_stop = (p->ensemble == MCGPU_NPT) ? std::min<int64_t>(p->steps, p->volume_interval)
                                            : p->steps;
@@ -1567,7 +1614,6 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
   }
   int rc = batch_boundary(b);
   if (rc) return rc;
-  if (b->slots[s].in_volume && (rc = finish_volume_pass(b))) return rc;
   if (b->slots[s].dirty) MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
   b->h_ctl[s] = BoxCtl{};
   b->h_ctl[s].status = kDone;
@@ -1587,25 +1633,18 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   const int stride = b->max_boxes + 1;
   int n[kGroups] = {};
   int total = 0;
-  for (int pass = 0; pass < 2 && total == 0; pass++) {
-    // nothing runnable while a volume pass is in flight: decide it first
-    if (pass == 1) {
-      if (!b->vol_inflight) break;
-      if ((rc = finish_volume_pass(b))) return rc;
+  b->launched.clear();
+  for (int s = 0; s < b->max_boxes; s++) {
+    const BatchSlot& S = b->slots[s];
+    if (S.used && b->h_ctl[s].status == kRunning) {
+      b->h_active[S.group * stride + 1 + n[S.group]++] = s;
+      b->launched.push_back(s);
+      b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
     }
-    for (int g = 0; g < kGroups; g++) n[g] = 0;
-    b->launched.clear();
-    for (int s = 0; s < b->max_boxes; s++) {
-      const BatchSlot& S = b->slots[s];
-      if (S.used && !S.in_volume && b->h_ctl[s].status == kRunning) {
-        b->h_active[S.group * stride + 1 + n[S.group]++] = s;
-        b->launched.push_back(s);
-        b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
-      }
-    }
-    total = (int)b->launched.size();
   }
+  total = (int)b->launched.size();
   if (total == 0) return MCGPU_OK;
+  const double tl = host_now();
   // set-up work queued so far (admissions, decided volume moves) first
   MCGPU_HIP_TRY(hipEventRecord(b->ev_ready, b->sstream));
   MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_ready, 0));
@@ -1664,7 +1703,17 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
                                hipMemcpyDeviceToHost, b->stream));
   b->pending = true;
   b->timing_pending = true;
+  b->t_launch += host_now() - tl;
   if (sync) return batch_boundary(b);
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_host_seconds(mcgpu_batch* b, double* wait, double* volume,
+                                        double* launch) {
+  if (!b) return MCGPU_ERR_INVALID;
+  if (wait) *wait = b->t_wait;
+  if (volume) *volume = b->t_volume;
+  if (launch) *launch = b->t_launch;
   return MCGPU_OK;
 }
 
@@ -1678,15 +1727,13 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   const BoxCtl& c = b->h_ctl[s];
   out->steps_done = c.steps_done;
   out->accepted = c.accepted;
-  out->volume_moves = b->slots[s].vol_moves;
-  out->volume_accepted = b->slots[s].vol_acc;
+  out->volume_moves = c.vol_moves;
+  out->volume_accepted = c.vol_acc;
   out->energy = c.energy;
   out->start_energy = c.start_energy;
   out->work_candidates = c.work_cand;
   out->work_sites = c.work_sites + c.work_self;
-  // a box in a volume move reports its edges before the move until decided
-  for (int k = 0; k < 3; k++)
-    out->box[k] = b->slots[s].in_volume ? b->slots[s].L_old[k] : b->h_table[s].bv.L[k];
+  for (int k = 0; k < 3; k++) out->box[k] = c.L[k];
   out->done = c.status == kDone;
   out->fault = c.fault ? c.fault : b->slots[s].fault;
   return MCGPU_OK;

@@ -178,10 +178,14 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       double kms = 0.0;
       int64_t nl = 0;
       if (mcgpu_batch_kernel_ms(s->batch, &kms, &nl) == MCGPU_OK) kernel_ms += kms * nl;
+      double tw = 0, tv = 0, tl = 0;
+      (void)mcgpu_batch_host_seconds(s->batch, &tw, &tv, &tl);
       std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu admitted=%zu done=%zu moves=%lld "
-                   "host s: admit %.2f run %.2f (kernels %.2f) retire %.2f\n",
+                   "host s: admit %.2f run %.2f (kernels %.2f; wait %.2f volume %.2f launch %.2f) "
+                   "retire %.2f\n",
                    now(), (long long)s->stats.ticks, resident.size(), next, completed,
-                   (long long)s->stats.total_moves, t_admit, t_run, kernel_ms / 1e3, t_retire);
+                   (long long)s->stats.total_moves, t_admit, t_run, kernel_ms / 1e3, tw, tv, tl,
+                   t_retire);
       std::fflush(stderr);
       next_log = now() + log_every;
     }
