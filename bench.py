@@ -191,6 +191,53 @@ def cpu_baseline(jobs, pool, budget_s: float):
                         f"single thread, {secs:.1f} s of loop time"))
 
 
+# VALU issue slots per site pair in the full-energy kernel's inner loop
+# (FP64 path, read from this build's gfx950 code object: 49 full-rate VALU
+# instructions -- 15 of them the three single wraps, 8 the rsq refinement --
+# plus v_rsq_f64 at quarter rate = 4 slots); with 256 CUs x 4 SIMDs x 16
+# lanes per cycle at 2.4 GHz this sets the pair-interaction roof.
+VALU_PER_PAIR_FP64 = 53
+VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
+
+
+def full_energy_measure():
+    """The full-system energy kernel (one wave per molecule) on the
+    20,000-molecule water/methanol mixture (rc 11 A, the NPT volume-move
+    workload) and the 4,000-molecule bicyclic box (rc 12 A): site-pair
+    interactions per second in FP64 and with FP32 pair terms, against the
+    VALU roof, and the FP32-pair relative error of the total."""
+    import mcgpu_amd as m
+    out = {}
+    for name in ("mixture20000", "bicyclic4000"):
+        hb = m.workload(name)
+        res = {}
+        for prec, tag in ((m.FP64, "fp64"), (m.MIXED, "mixed")):
+            db = m.DeviceBox(hb, precision=prec)
+            db.neighbor(m.NEIGHBOR_CELL)
+            db.enable_timing(True)
+            e = db.system_energy()
+            best = float("inf")
+            for _ in range(5):
+                e = db.system_energy()
+                best = min(best, db.last_kernel_ms())
+            res[tag] = (e, db.system_pairs(), best)
+            db.close()
+        e64, pairs, ms64 = res["fp64"]
+        e32, _, ms32 = res["mixed"]
+        roof = VALU_LANE_OPS_PER_S / VALU_PER_PAIR_FP64
+        out[name] = {
+            "site_pairs": int(pairs),
+            "fp64_ms": ms64,
+            "fp64_pairs_per_s": pairs / (ms64 / 1e3),
+            "fp64_frac_of_valu_roof": pairs / (ms64 / 1e3) / roof,
+            "mixed_ms": ms32,
+            "mixed_pairs_per_s": pairs / (ms32 / 1e3),
+            "mixed_rel_err": abs(e32 - e64) / abs(e64),
+        }
+    out["valu_roof_pairs_per_s"] = VALU_LANE_OPS_PER_S / VALU_PER_PAIR_FP64
+    return out
+
+
 def pmc_traffic():
     """HBM bytes per launch of the batch kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
@@ -272,6 +319,9 @@ def main() -> int:
     print(f"# plain FP64 path: {plain_rate:.4g} moves/s over {plain_jobs} jobs",
           file=sys.stderr, flush=True)
 
+    full = full_energy_measure()
+    print(f"# full energy: {json.dumps(full)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(timed, pool, args.cpu_seconds)
@@ -319,6 +369,7 @@ def main() -> int:
             # capacity serving), to normalise per-process counter totals
             "graph_launches_in_run": int(wst.ticks + st.ticks + cst.ticks),
         },
+        "full_energy": full,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -26,6 +26,7 @@
 #include "cells.h"
 #include "eval.h"
 #include "fastmath.h"
+#include "fullenergy.h"
 #include "mcgpu_host.h"
 #include "wave.h"
 
@@ -519,202 +520,13 @@ __global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k
   c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
 }
 
-// Full-system energy (SimulationStep.cpp:40-71: the sum over i of E_i(i),
-// partners j > i) of every listed box, one wave per molecule i: the lanes
-// scan the candidates (the cells around i's first primary, or all j > i),
-// test the primary cutoff (SimulationStep.cpp:317-330), compact the hits
-// into the wave's LDS list by ballot, then sum the site pairs of the hits
-// (each lane one (hit, site b) item against every site a of i), and the
-// wave sum goes to partial[i].  Waves never wait for each other: a
-// throughput kernel that keeps many waves per CU in flight.
-constexpr int kWaveHits = 256;
-
-struct WaveEval {
-  double xa[kMaxSites], ya[kMaxSites], za[kMaxSites];  // sites of molecule i
-  int32_t pa[kMaxSites];                               // their parameter entries
-  int32_t hit_start[kWaveHits];
-  int32_t hit_param[kWaveHits];
-  int32_t hit_len[kWaveHits];
-};
-
-template <bool MIXED>
-__device__ __forceinline__ double wave_hits_energy(const BoxView& bv, const WaveEval& W,
-                                                   int len_i, int nh,
-                                                   const SiteParam64* P64,
-                                                   const SiteParam32* P32) {
-  const int M = bv.max_len;
-  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
-  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
-  double e = 0.0;
-  for (int u = wlane(); u < nh * M; u += 64) {
-    const int k = div_small(u, magic);
-    const int b = u - k * M;
-    if (b >= W.hit_len[k]) continue;
-    const int sb = W.hit_start[k] + b;
-    const int pb = W.hit_param[k] + b;
-    const SiteParam64 B = P64[pb];
-    if (B.real == 0.0) continue;
-    const double xb = bv.x[sb], yb = bv.y[sb], zb = bv.z[sb];
-    for (int a = 0; a < len_i; a++) {
-      const int pa = W.pa[a];
-      const SiteParam64 A = P64[pa];
-      if (A.real == 0.0) continue;
-      const double dx = wrap1(xb - W.xa[a], L0);
-      const double dy = wrap1(yb - W.ya[a], L1);
-      const double dz = wrap1(zb - W.za[a], L2);
-      if (MIXED) {
-        const SiteParam32 A32 = P32[pa], B32 = P32[pb];
-        e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e, B.s,
-                           B.e);
-      } else {
-        e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
-      }
-    }
-  }
-  return e;
-}
-
-template <bool MIXED>
-__device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEval& W, int i,
-                                                       const SiteParam64* P64,
-                                                       const SiteParam32* P32) {
-  const int lane = wlane();
-  int start, len, param, psi, pci;
-  if (bv.uni_len > 0) {
-    len = bv.uni_len;
-    start = i * len;
-    param = bv.uni_param;
-    psi = i;
-    pci = 1;
-  } else {
-    const MolMeta m = bv.meta[i];
-    start = m.start;
-    len = m.len;
-    param = m.param;
-    psi = bv.single_primary ? i : m.pstart;
-    pci = bv.single_primary ? 1 : bv.mol_pidx_count[i];
-  }
-  if (lane < len) {
-    W.xa[lane] = bv.x[start + lane];
-    W.ya[lane] = bv.y[start + lane];
-    W.za[lane] = bv.z[start + lane];
-    W.pa[lane] = param + lane;
-  }
-  // the primaries of i (at most kMaxSites) as lane values
-  double pix = 0.0, piy = 0.0, piz = 0.0;
-  if (lane < pci) {
-    pix = bv.px[psi + lane];
-    piy = bv.py[psi + lane];
-    piz = bv.pz[psi + lane];
-  }
-  __builtin_amdgcn_wave_barrier();
-  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
-  // candidate source
-  const CellGrid& g = bv.cells;
-  const bool cells = bv.mode == MCGPU_NB_CELL && g.overflow[0] == 0;
-  CellRange r{};
-  int ncell = 0, incl = 0, nitems = bv.n_mol - (i + 1);
-  if (cells) {
-    int co[3];
-    co[0] = cell_coord(readlane_f64(pix, 0), g.inv_w[0], g.n[0]);
-    co[1] = cell_coord(readlane_f64(piy, 0), g.inv_w[1], g.n[1]);
-    co[2] = cell_coord(readlane_f64(piz, 0), g.inv_w[2], g.n[2]);
-    r = make_range(g, co, co);  // at most 3 x 3 x 3 cells
-    ncell = r.total();
-    int n = 0;
-    if (lane < ncell) {
-      n = g.count[r.cell(g, lane)];
-      n = n < g.cap ? n : g.cap;
-    }
-    incl = wave_incl_scan_i32(n);
-    nitems = readlane_i32(incl, 63);
-  }
-  double e = 0.0;
-  int nh = 0;
-  for (int base = 0; base < nitems; base += 64) {
-    const int item = base + lane;
-    int j = -1;
-    if (cells) {
-      // cell holding `item`: the number of cells whose inclusive prefix is
-      // <= item (all lanes take part in the lane reads)
-      int lo = 0;
-      for (int step = 32; step > 0; step >>= 1) {
-        const int probe = lo + step - 1;
-        const int v = __shfl(incl, probe < 63 ? probe : 63);
-        if (probe < ncell && v <= item) lo += step;
-      }
-      const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
-      if (item < nitems) j = g.slots[r.cell(g, lo) * g.cap + (item - (lo > 0 ? before : 0))];
-    } else if (item < nitems) {
-      j = i + 1 + item;
-    }
-    bool hit = false;
-    int hs = 0, hp = 0, hl = 0;
-    if (j > i) {
-      const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
-      const int pcj = bv.single_primary ? 1 : bv.mol_pidx_count[j];
-      for (int q = 0; q < pcj; q++) {
-        const double qx = bv.px[psj + q], qy = bv.py[psj + q], qz = bv.pz[psj + q];
-        for (int p = 0; p < pci; p++) {
-          const double dx = wrap1(qx - readlane_f64(pix, p), L0);
-          const double dy = wrap1(qy - readlane_f64(piy, p), L1);
-          const double dz = wrap1(qz - readlane_f64(piz, p), L2);
-          hit |= dx * dx + dy * dy + dz * dz <= bv.cutoff2;
-        }
-      }
-      if (hit) {
-        if (bv.uni_len > 0) {
-          hl = bv.uni_len;
-          hs = j * hl;
-          hp = bv.uni_param;
-        } else {
-          const MolMeta m = bv.meta[j];
-          hs = m.start;
-          hl = m.len;
-          hp = m.param;
-        }
-      }
-    }
-    const unsigned long long bal = __ballot(hit);
-    const int cnt = __popcll(bal);
-    if (nh + cnt > kWaveHits) {  // wave-uniform
-      e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32);
-      __builtin_amdgcn_wave_barrier();
-      nh = 0;
-    }
-    if (hit) {
-      const int pos = nh + wave_rank(bal);
-      W.hit_start[pos] = hs;
-      W.hit_param[pos] = hp;
-      W.hit_len[pos] = hl;
-    }
-    nh += cnt;
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32);
-  __builtin_amdgcn_wave_barrier();
-  return wave_sum_f64(e);
-}
-
 __global__ __launch_bounds__(256) void k_energy_waves(const BatchBox* table,
                                                       const VolItem* items) {
   __shared__ WaveEval ws[4];
   __shared__ SiteParam64 p64[kParamCap];
   __shared__ SiteParam32 p32[kParamCap];
   const BatchBox& B = table[items[blockIdx.y].slot];
-  const BoxView& bv = B.bv;
-  for (int k = threadIdx.x; k < bv.n_param; k += 256) {
-    p64[k] = bv.p64[k];
-    p32[k] = bv.p32[k];
-  }
-  __syncthreads();
-  const int w = threadIdx.x >> 6;
-  const bool mixed = B.precision == MCGPU_MIXED;
-  for (int i = blockIdx.x * 4 + w; i < bv.n_mol; i += gridDim.x * 4) {
-    const double e = mixed ? wave_molecule_energy<true>(bv, ws[w], i, p64, p32)
-                           : wave_molecule_energy<false>(bv, ws[w], i, p64, p32);
-    if (wlane() == 0) B.partial[i] = e;
-  }
+  full_energy_block(B.bv, B.precision == MCGPU_MIXED, B.partial, nullptr, ws, p64, p32);
 }
 
 // Fixed-order sum of entry blockIdx.x's per-molecule partials into

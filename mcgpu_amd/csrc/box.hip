@@ -18,6 +18,7 @@
 #include "cells.h"
 #include "eval.h"
 #include "fastmath.h"
+#include "fullenergy.h"
 
 namespace mcgpu {
 
@@ -146,23 +147,14 @@ __global__ __launch_bounds__(NT) void k_verlet_build(BoxView bv, double rv2) {
   }
 }
 
-template <int NT, bool MIXED>
-__global__ __launch_bounds__(NT) void k_system(BoxView bv, double* partial,
-                                               double* pairs) {
-  __shared__ EvalShared<NT> sh;
-  load_params<NT>(bv, sh);
-  for (int mol = blockIdx.x; mol < bv.n_mol; mol += gridDim.x) {
-    load_molecule<NT>(bv, mol, sh);
-    __syncthreads();
-    double eo, en;
-    unsigned long long np = 0;
-    eval_molecule<NT, MIXED>(bv, mol, mol + 1, false, sh, eo, en, &np);
-    if (threadIdx.x == 0) {
-      partial[mol] = eo;
-      pairs[mol] = (double)np;
-    }
-    __syncthreads();
-  }
+// Full-system energy of the box (SimulationStep.cpp:40-71), one wave per
+// molecule (see fullenergy.h): partial[i] = E_i(i), pairs[i] = site pairs.
+__global__ __launch_bounds__(256) void k_system_waves(BoxView bv, int mixed, double* partial,
+                                                      double* pairs) {
+  __shared__ WaveEval ws[4];
+  __shared__ SiteParam64 p64[kParamCap];
+  __shared__ SiteParam32 p32[kParamCap];
+  full_energy_block(bv, mixed != 0, partial, pairs, ws, p64, p32);
 }
 
 // Fixed-order sum of n partials into out[0] (and n counts into out[1]).
@@ -227,22 +219,6 @@ int launch_cells_build(const BoxView& bv, hipStream_t s) {
   hipLaunchKernelGGL(k_cells_assign, dim3((bv.n_mol + 255) / 256), dim3(256), 0,
                      s, bv);
   hipLaunchKernelGGL(k_cells_sort, dim3((ncell + 3) / 4), dim3(256), 0, s, g, ncell);
-  MCGPU_HIP_TRY(hipGetLastError());
-  return MCGPU_OK;
-}
-
-int launch_system_energy(const BoxView& bv, int precision, double* d_partial,
-                         double* d_pairs, double* d_out2, hipStream_t s) {
-  const int n = bv.n_mol;
-  const int grid = std::min(n, 8192);
-  if (precision == MCGPU_MIXED)
-    hipLaunchKernelGGL((k_system<64, true>), dim3(grid), dim3(64), 0, s, bv,
-                       d_partial, d_pairs);
-  else
-    hipLaunchKernelGGL((k_system<64, false>), dim3(grid), dim3(64), 0, s, bv,
-                       d_partial, d_pairs);
-  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, s, d_partial, d_pairs, n,
-                     d_out2);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
@@ -843,14 +819,10 @@ extern "C" int mcgpu_trial_move(mcgpu_box* b, int32_t mol,
 extern "C" int mcgpu_system_energy(mcgpu_box* b, double* energy) {
   if (!b || !energy) return MCGPU_ERR_INVALID;
   const int n = b->bv.n_mol;
-  const int grid = std::min(n, 8192);
+  const int grid = std::min((n + 3) / 4, 2048);
   if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
-  if (b->precision == MCGPU_MIXED)
-    hipLaunchKernelGGL((k_system<64, true>), dim3(grid), dim3(64), 0, b->stream,
-                       b->bv, b->d_partial, b->d_pairs);
-  else
-    hipLaunchKernelGGL((k_system<64, false>), dim3(grid), dim3(64), 0, b->stream,
-                       b->bv, b->d_partial, b->d_pairs);
+  hipLaunchKernelGGL(k_system_waves, dim3(grid), dim3(256), 0, b->stream, b->bv,
+                     (int)(b->precision == MCGPU_MIXED), b->d_partial, b->d_pairs);
   MCGPU_HIP_TRY(hipGetLastError());
   if (b->ev1) MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, b->stream, b->d_partial,

@@ -59,8 +59,6 @@ int ensure_device();
 // no synchronisation).
 int launch_refresh_primary(const BoxView& bv, int n_pidx, hipStream_t s);
 int launch_cells_build(const BoxView& bv, hipStream_t s);
-int launch_system_energy(const BoxView& bv, int precision, double* d_partial,
-                         double* d_pairs, double* d_out2, hipStream_t s);
 int launch_volume_scale(const BoxView& bv, double f, hipStream_t s);
 
 }  // namespace mcgpu
