@@ -56,6 +56,38 @@ int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y, double** z);
 int mcgpu_state_write(mcgpu_hostbox* h, const char* path, int64_t step);
 int mcgpu_state_read(const char* path, mcgpu_hostbox** out, int64_t* step);
 
+/* .config runs (metrosim's z-matrix input route).  Replaces loadBoxData's
+ * Configuration branch (FileUtilities.cpp:19-110): ConfigScanner
+ * (ConfigScanner.cpp:20-265), OplsScanner (OplsScanner.cpp:18-101),
+ * ZmatrixScanner (ZmatrixScanner.cpp:21-215), buildMoleculeXYZ
+ * (MathLibrary.cpp:751-902), buildBoxData + generatefccBox
+ * (FileUtilities.cpp:285-642), then keepMoleculeInBox per molecule
+ * (Simulation.cpp:70-72) and addPrimaryIndexes (SimBoxBuilder.cpp:282-299).
+ * Relative .par / z-matrix paths are opened from the current directory, as
+ * the reference does.  The box holds the requested number of molecules. */
+typedef struct mcgpu_config_info {
+  int64_t steps;          /* "steps" */
+  int32_t n_molecules;    /* "molecules" */
+  int32_t n_types;        /* molecule types in the z-matrix */
+  int32_t random_seed;    /* "random-seed" (the MC seed, also in the box) */
+  int32_t volume_interval;/* "volume-interval" (0 if absent) */
+  int32_t ewald_radius;   /* "ewald-radius" (0 if absent) */
+  int32_t reserved;
+  double pressure;        /* "pressure" (0 if absent) */
+  char sim_name[256];     /* "sim-name" */
+  char strategy[64];      /* "strategy" */
+  char ensemble[64];      /* "ensemble" */
+  char state_output[1024];/* "state-output" */
+  char pdb_output[1024];  /* "pdb-output" */
+} mcgpu_config_info;
+
+/* Returns MCGPU_ERR_INVALID (with the message in mcgpu_last_error) for an
+ * unreadable or malformed file, an unknown key, an OPLS type missing from
+ * the .par file, or a primary-atom line whose group count differs from the
+ * z-matrix's molecule types.  info may be NULL. */
+int mcgpu_config_read(const char* path, mcgpu_hostbox** out,
+                      mcgpu_config_info* info);
+
 /* The reference's LRC (SerialCalcs.cpp:37-108) and the K-species form. */
 int mcgpu_lrc_reference(const mcgpu_box_desc* d, double* e);
 int mcgpu_lrc_species(const mcgpu_box_desc* d, double* e);

@@ -112,6 +112,17 @@ class ServerStats(C.Structure):
     ]
 
 
+class ConfigInfo(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("n_molecules", C.c_int32), ("n_types", C.c_int32),
+        ("random_seed", C.c_int32), ("volume_interval", C.c_int32),
+        ("ewald_radius", C.c_int32), ("reserved", C.c_int32),
+        ("pressure", C.c_double), ("sim_name", C.c_char * 256),
+        ("strategy", C.c_char * 64), ("ensemble", C.c_char * 64),
+        ("state_output", C.c_char * 1024), ("pdb_output", C.c_char * 1024),
+    ]
+
+
 class McgpuError(RuntimeError):
     pass
 

@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BoxDesc, BoxStats, JobResult, MoveResult, PlainResult,
+from . import (BoxDesc, BoxStats, ConfigInfo, JobResult, MoveResult, PlainResult,
                RunParams, ServerConfig, ServerStats, WorkloadParams, check,
                lib)
 
@@ -56,6 +56,24 @@ class HostBox:
         check(lib().mcgpu_state_read(str(path).encode(), C.byref(h),
                                      C.byref(step)), "mcgpu_state_read")
         return cls(h)
+
+    @classmethod
+    def from_config(cls, path: str):
+        """Loads a metrosim .config run (z-matrix + OPLS .par input); returns
+        (box, settings) where settings holds the run's other keys."""
+        h = C.c_void_p()
+        info = ConfigInfo()
+        check(lib().mcgpu_config_read(str(path).encode(), C.byref(h),
+                                      C.byref(info)), "mcgpu_config_read")
+        settings = dict(
+            steps=info.steps, n_molecules=info.n_molecules, n_types=info.n_types,
+            random_seed=info.random_seed, volume_interval=info.volume_interval,
+            ewald_radius=info.ewald_radius, pressure=info.pressure,
+            sim_name=info.sim_name.decode(), strategy=info.strategy.decode(),
+            ensemble=info.ensemble.decode(),
+            state_output=info.state_output.decode(),
+            pdb_output=info.pdb_output.decode())
+        return cls(h), settings
 
     def write_state(self, path: str, step: int = 0) -> None:
         check(lib().mcgpu_state_write(self._h, str(path).encode(),

@@ -872,10 +872,12 @@ extern "C" int mcgpu_debug_math(const double* x, int32_t n, double* s, double* c
   if (n == 0) return MCGPU_OK;
   double* d = nullptr;
   MCGPU_HIP_TRY(hipMalloc(&d, sizeof(double) * 4 * (size_t)n));
-  hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, 0, d, n,
-                     d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
-  hipError_t err = hipDeviceSynchronize();
+  hipError_t err = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, 0, d, n,
+                       d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
+    err = hipDeviceSynchronize();
+  }
   if (err == hipSuccess) err = hipMemcpy(s, d + n, sizeof(double) * n, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(c, d + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(e, d + 3 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost);

@@ -74,10 +74,18 @@ HostBox HostBox::from_desc(const mcgpu_box_desc& d) {
   return b;
 }
 
+double lrc_species(const mcgpu_box_desc& d);
+
 // SerialCalcs.cpp:37-108 -- the reference's tail correction: "solvent 1"
 // and "solvent 2" are molecules 0 and 1, each counted N/2 times.  The
 // arithmetic order is kept so start energies match the reference exactly.
+// Its pair loop indexes each molecule's coefficients with the other
+// molecule's site index (SerialCalcs.cpp:98-104), so it is defined only when
+// molecules 0 and 1 have the same length; other boxes (the reference reads
+// past its arrays there) take the species form below.
 double lrc_reference(const mcgpu_box_desc& d) {
+  const int32_t ma = 0, mb = d.n_molecules > 1 ? 1 : 0;
+  if (d.mol_len[ma] != d.mol_len[mb]) return lrc_species(d);
   const double PI_ = 3.14159265;  // MathLibrary.h:51
   double ecut = 0.0;
   const double vnew = d.box[0] * d.box[1] * d.box[2];
@@ -85,7 +93,6 @@ double lrc_reference(const mcgpu_box_desc& d) {
   const double rc9 = std::pow(rc3, 3);
   const double nmol1 = (double)(d.n_molecules / 2);
   const double nmol2 = (double)(d.n_molecules / 2);
-  const int32_t ma = 0, mb = d.n_molecules > 1 ? 1 : 0;
   auto coeffs = [&](int32_t mol, std::vector<double>& c6, std::vector<double>& c12) {
     for (int32_t i = 0; i < d.mol_len[mol]; i++) {
       const int32_t s = d.mol_start[mol] + i;

@@ -168,6 +168,12 @@ double orc_system_energy(const orc_box* b) {
 /* SerialCalcs.cpp:37-108, restated with the same operation order.  Uses the
  * first two molecules as "solvent 1" and "solvent 2" and N/2 of each. */
 double orc_lrc_reference(const orc_box* b) {
+  /* The pair loop below indexes each molecule's coefficients with the other
+   * molecule's site index (SerialCalcs.cpp:98-104): defined only for equal
+   * lengths of molecules 0 and 1.  The reference reads past its arrays
+   * otherwise, which nothing can match; such boxes take the species form
+   * (parity with the reference unpinned there). */
+  if (b->mol_len[0] != b->mol_len[b->n_mol > 1 ? 1 : 0]) return orc_lrc_species(b);
   const double PI_ = 3.14159265; /* MathLibrary.h:51 */
   double ecut = 0.0;
   const double vnew = b->L[0] * b->L[1] * b->L[2];
