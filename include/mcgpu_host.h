@@ -52,9 +52,14 @@ int mcgpu_hostbox_info(mcgpu_hostbox* h, double* temperature,
 /* Writable views of the coordinate arrays (for loading final states). */
 int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y, double** z);
 
+/* Sets the box edges (e.g. after NPT volume moves, before a state write). */
+int mcgpu_hostbox_set_edges(mcgpu_hostbox* h, const double* edges3);
+
 /* .state files, full precision (%.17g) so a reload is bit exact. */
 int mcgpu_state_write(mcgpu_hostbox* h, const char* path, int64_t step);
 int mcgpu_state_read(const char* path, mcgpu_hostbox** out, int64_t* step);
+/* PDB snapshot in the reference's column layout (Simulation.cpp:362-417). */
+int mcgpu_pdb_write(mcgpu_hostbox* h, const char* path);
 
 /* .config runs (metrosim's z-matrix input route).  Replaces loadBoxData's
  * Configuration branch (FileUtilities.cpp:19-110): ConfigScanner
@@ -88,7 +93,9 @@ typedef struct mcgpu_config_info {
 int mcgpu_config_read(const char* path, mcgpu_hostbox** out,
                       mcgpu_config_info* info);
 
-/* The reference's LRC (SerialCalcs.cpp:37-108) and the K-species form. */
+/* The reference's LRC (SerialCalcs.cpp:37-108) where it is defined -- its
+ * pair loop needs molecules 0 and 1 of equal length; other boxes get the
+ * K-species form -- and the K-species form itself. */
 int mcgpu_lrc_reference(const mcgpu_box_desc* d, double* e);
 int mcgpu_lrc_species(const mcgpu_box_desc* d, double* e);
 

@@ -188,3 +188,33 @@ extern "C" int mcgpu_state_read(const char* path, mcgpu_hostbox** out,
   *out = h;
   return MCGPU_OK;
 }
+
+// PDB snapshot in the reference's column layout (Simulation.cpp:362-417):
+// one ATOM row per site ("ATOM", site number, name, "UNK", molecule number,
+// x y z with 3 decimals), TER after each molecule, END at the close.
+extern "C" int mcgpu_pdb_write(mcgpu_hostbox* h, const char* path) {
+  if (!h || !path) return MCGPU_ERR_INVALID;
+  const HostBox& b = mcgpu::hostbox_ref(h);
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    mcgpu::set_error(std::string("mcgpu_pdb_write: cannot open ") + path);
+    return MCGPU_ERR_INVALID;
+  }
+  std::fprintf(f, "REMARK Created by MCGPU\n");
+  for (int m = 0; m < b.n_mol(); m++) {
+    for (int k = 0; k < b.mol_len[m]; k++) {
+      const int s = b.mol_start[m] + k;
+      const char* nm = b.site_name.empty() ? "X" : b.site_name[s].c_str();
+      std::fprintf(f, "%-6s%5d%3s%6s%6d%12.3f%8.3f%8.3f\n", "ATOM", s + 1, nm,
+                   "UNK", m + 1, b.x[s], b.y[s], b.z[s]);
+    }
+    std::fprintf(f, "TER\n");
+  }
+  std::fprintf(f, "END\n");
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) {
+    mcgpu::set_error(std::string("mcgpu_pdb_write: write failed on ") + path);
+    return MCGPU_ERR_INVALID;
+  }
+  return MCGPU_OK;
+}

@@ -457,6 +457,17 @@ extern "C" int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y,
   return MCGPU_OK;
 }
 
+extern "C" int mcgpu_hostbox_set_edges(mcgpu_hostbox* h, const double* edges3) {
+  if (!h || !edges3) return MCGPU_ERR_INVALID;
+  for (int k = 0; k < 3; k++)
+    if (!(edges3[k] > 0)) {
+      mcgpu::set_error("mcgpu_hostbox_set_edges: edge <= 0");
+      return MCGPU_ERR_INVALID;
+    }
+  for (int k = 0; k < 3; k++) h->b.L[k] = edges3[k];
+  return MCGPU_OK;
+}
+
 namespace mcgpu {
 HostBox& hostbox_ref(mcgpu_hostbox* h) { return h->b; }
 mcgpu_hostbox* hostbox_new(HostBox&& b) {

@@ -1,0 +1,94 @@
+"""The metrosim command line on the GPU, driven the way the reference's
+integration tests drive theirs (TestUtil.cpp:4-130): write a .config, run
+`metrosim <config> -s --name X -i 10000`, read Final-Energy from X.results
+(text after column 15).  The energy must be the reference's own full run of
+the same .config (reference_runs.json, from its CPU path compiled here), to
+the 6 significant digits the .results file carries and to the 3 decimals
+the console prints.  The final .state file must reload to a box whose fresh
+full energy plus LRC equals that Final-Energy (no drift in the running
+energy)."""
+import json
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mcgpu_amd as m
+from config_cases import CASES, CFG, write_config
+
+pytestmark = pytest.mark.gpu
+METROSIM = Path(__file__).resolve().parents[1] / "mcgpu_amd" / "bin" / "metrosim"
+RUNS = json.loads((CFG / "reference_runs.json").read_text())
+
+
+def energy_result(path: Path) -> float:
+    """getEnergyResult (TestUtil.cpp:112-130)."""
+    for line in path.read_text().splitlines():
+        if "Final-Energy" in line:
+            return float(line[15:])
+    return -1.0
+
+
+def metrosim(args, cwd, timeout=300):
+    r = subprocess.run([str(METROSIM), *args], cwd=cwd, capture_output=True,
+                       text=True, timeout=timeout)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    return r
+
+
+@pytest.mark.parametrize("name", ["t3pdim_c", "meoh500_1"])
+def test_metrosim_reproduces_reference_final_energy(name, tmp_path):
+    case = CASES[name]
+    cfg = write_config(tmp_path / f"{name}.config", *case)
+    r = metrosim([str(cfg), "-s", "--name", name, "-i", "10000"], tmp_path)
+    ref = RUNS[name]
+    got = energy_result(tmp_path / f"{name}.results")
+    assert got == float("%.6g" % ref["final_energy"])
+    assert f"Final Energy: {ref['final_energy']:.3f}" in r.stdout
+    assert f"Accepted Moves: {ref['accepted']}" in r.stdout
+    if ref["reference_meets_expected"]:
+        assert abs(got - ref["expected"]) <= ref["margin"]
+    # the running energy against a fresh evaluation of the final state
+    st = tmp_path / f"{name}_{case[3]}.state"
+    assert st.exists() and (tmp_path / f"{name}.pdb").exists()
+    hb = m.HostBox.from_state(str(st))
+    db = m.DeviceBox(hb)
+    e = db.system_energy() + ref["lrc"]
+    final = float(re.search(r"Final Energy: (\S+)", r.stdout).group(1))
+    assert abs(e - final) < 2e-3
+
+
+def test_metrosim_state_interval_and_resume(tmp_path):
+    case = CASES["meoh_t3p256"]
+    cfg = write_config(tmp_path / "run.config", *case)
+    metrosim([str(cfg), "-p", "-n", "3000", "-I", "1000", "--name", "snap"], tmp_path)
+    for step in (1000, 2000, 3000):
+        assert (tmp_path / f"snap_{step}.state").exists()
+    r = metrosim([str(tmp_path / "snap_3000.state"), "-n", "500", "--name", "more",
+                  "-I", "-1"], tmp_path)
+    assert "starting from step 3000" in r.stdout
+    assert "Starting-Step = 3000" in (tmp_path / "more.results").read_text()
+    assert not list(tmp_path.glob("more_*.state"))
+
+
+def test_metrosim_npt_run(tmp_path):
+    case = CASES["t3pdimone_1"]
+    cfg = write_config(tmp_path / "npt.config", *case,
+                       extra="ensemble=npt\npressure=1.0\nvolume-interval=500\n")
+    r = metrosim([str(cfg), "-n", "20000", "--name", "npt"], tmp_path)
+    vm = re.search(r"Volume Moves: (\d+) accepted of (\d+)", r.stdout)
+    assert vm and int(vm.group(2)) >= 39
+    st = tmp_path / "npt_20000.state"
+    hb = m.HostBox.from_state(str(st))
+    a = hb.arrays()
+    assert a["box"][0] != case[1]          # the volume moved
+    db = m.DeviceBox(hb)
+    import ctypes as C
+    d = hb.desc()
+    lrc = C.c_double()
+    m.check(m.lib().mcgpu_lrc_reference(C.byref(d), C.byref(lrc)))
+    final = float(re.search(r"Final Energy: (\S+)", r.stdout).group(1))
+    assert abs(db.system_energy() + lrc.value - final) < 2e-3
