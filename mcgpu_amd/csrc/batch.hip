@@ -1321,6 +1321,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   v.mode = MCGPU_NB_BRUTE;
   for (int k = 0; k < 3; k++) v.L[k] = d->box[k];
   v.cutoff2 = d->cutoff * d->cutoff;
+  v.reach2 = reach2_of(pt.reach);
   v.kT = d->kT;
   v.cells.count = (int32_t*)(base + L.ccount);
   v.cells.slots = (int32_t*)(base + L.cslots);

@@ -245,6 +245,16 @@ int ensure_device() {
   return MCGPU_OK;
 }
 
+double molecule_reach(const double* x, const double* y, const double* z, int len,
+                      int pf) {
+  double r2 = 0.0;
+  for (int k = 0; k < len; k++) {
+    const double dx = x[k] - x[pf], dy = y[k] - y[pf], dz = z[k] - z[pf];
+    r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+  }
+  return std::sqrt(r2);
+}
+
 ParamTable build_param_table(const mcgpu_box_desc& d) {
   ParamTable t;
   t.mol_param.resize(d.n_molecules);
@@ -293,8 +303,9 @@ ParamTable build_param_table(const mcgpu_box_desc& d) {
     if (s0 != m * t.uni_len || len != t.uni_len || it->second != t.uni_param ||
         pf != t.uni_p0)
       t.uniform = false;
-    // primary spread (rigid molecules: fixed)
+    // primary spread and site reach (rigid molecules: fixed)
     const int32_t p0 = d.primary_index[d.mol_pidx_start[m]];
+    t.reach = std::max(t.reach, molecule_reach(d.x + s0, d.y + s0, d.z + s0, len, pf));
     for (int32_t k = 1; k < d.mol_pidx_count[m]; k++) {
       const int32_t p = d.primary_index[d.mol_pidx_start[m] + k];
       const double dx = d.x[p] - d.x[p0], dy = d.y[p] - d.y[p0], dz = d.z[p] - d.z[p0];
@@ -341,6 +352,8 @@ struct mcgpu_box {
   int64_t last_pairs = 0;
   float last_ms = 0.0f;
   std::vector<int32_t> h_mol_start, h_mol_len;
+  std::vector<int16_t> h_pf;  // first primary's offset within each molecule
+  double reach = 0.0;         // see ParamTable::reach (only ever grows)
 };
 
 namespace {
@@ -498,6 +511,15 @@ int launch_move(mcgpu_box* b, const MoveArgs& a) {
   return MCGPU_OK;
 }
 
+// Caller-supplied coordinates may change a molecule's shape: keep the
+// box's site reach an upper bound of every molecule's.
+void note_reach(mcgpu_box* b, double reach) {
+  if (reach > b->reach) {
+    b->reach = reach;
+    b->bv.reach2 = reach2_of(reach);
+  }
+}
+
 int fill_coords(mcgpu_box* b, int32_t mol, const double* xyz, MoveArgs& a) {
   const int len = b->h_mol_len[mol];
   for (int k = 0; k < 3 * len; k++) {
@@ -507,6 +529,7 @@ int fill_coords(mcgpu_box* b, int32_t mol, const double* xyz, MoveArgs& a) {
     }
     a.nxyz[k] = xyz[k];
   }
+  note_reach(b, molecule_reach(xyz, xyz + len, xyz + 2 * len, len, b->h_pf[mol]));
   return MCGPU_OK;
 }
 
@@ -556,7 +579,11 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   b->n_pidx = d->n_primary;
   b->h_mol_start.assign(d->mol_start, d->mol_start + d->n_molecules);
   b->h_mol_len.assign(d->mol_len, d->mol_len + d->n_molecules);
+  b->h_pf.resize(d->n_molecules);
+  for (int32_t m = 0; m < d->n_molecules; m++) b->h_pf[m] = pt.meta[m].p0;
+  b->reach = pt.reach;
   BoxView& v = b->bv;
+  v.reach2 = reach2_of(pt.reach);
   v.n_mol = d->n_molecules;
   v.n_site = d->n_sites;
   v.max_len = pt.max_len;
@@ -729,6 +756,10 @@ extern "C" int mcgpu_upload_coords(mcgpu_box* b, const double* x,
                                    const double* y, const double* z) {
   if (!b || !x || !y || !z) return MCGPU_ERR_INVALID;
   const size_t n = b->bv.n_site;
+  for (int32_t m = 0; m < b->bv.n_mol; m++) {
+    const int32_t s0 = b->h_mol_start[m];
+    note_reach(b, molecule_reach(x + s0, y + s0, z + s0, b->h_mol_len[m], b->h_pf[m]));
+  }
   MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.x, x, 8 * n, hipMemcpyHostToDevice, b->stream));
   MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.y, y, 8 * n, hipMemcpyHostToDevice, b->stream));
   MCGPU_HIP_TRY(hipMemcpyAsync(b->bv.z, z, 8 * n, hipMemcpyHostToDevice, b->stream));

@@ -42,12 +42,21 @@ struct ParamTable {
   std::vector<MolMeta> meta;
   int32_t single_primary = 1;
   double spread = 0.0;  // max distance of a primary from the first primary
+  double reach = 0.0;   // max distance of a site from the first primary
   int32_t max_len = 0;
   // every molecule alike (see BoxView::uni_len); zeros otherwise
   bool uniform = true;
   int32_t uni_len = 0, uni_param = 0, uni_p0 = 0;
 };
 ParamTable build_param_table(const mcgpu_box_desc& d);
+
+// BoxView::reach2 for a largest site distance `reach` from the first primary.
+inline double reach2_of(double reach) { return 2.0 * reach + 1e-6; }
+
+// Largest distance of a site of one molecule from its first primary (site
+// pf of the molecule); xyz = x block, y block, z block of len sites each.
+double molecule_reach(const double* x, const double* y, const double* z, int len,
+                      int pf);
 
 // Cell grid geometry for a box: cells of width >= rc + 2 * spread.
 void cell_geometry(const double L[3], double cutoff, double spread, int n[3],

@@ -108,6 +108,10 @@ struct BoxView {
   int32_t pad0;
   double L[3];
   double cutoff2;
+  // Twice the largest distance of a site from its molecule's first primary,
+  // plus 1e-6 A for rounding: bounds, per axis, how far a site pair's
+  // displacement lies from the displacement of the two first primaries.
+  double reach2;
   double kT;
   CellGrid cells;
   VerletList verlet;
@@ -123,6 +127,36 @@ __device__ __forceinline__ double wrap1(double d, double L) {
   const double h = 0.5 * L;
   const double k = d < -h ? 1.0 : (d > h ? -1.0 : 0.0);
   return fma(k, L, d);
+}
+
+// The wrap of wrap1 for every site pair of a molecule pair at once, from
+// the raw displacement d of their first primaries along one axis (h = L/2,
+// reach = BoxView::reach2): every site pair's displacement lies within
+// reach of d, so it is 0 (no wrap) when d stays within h - reach of zero,
+// 1 (+L) when d < -(h + reach), 2 (-L) when d > h + reach, and 3 (site
+// pairs may wrap differently) otherwise.
+__device__ __forceinline__ int image_class(double d, double h, double reach) {
+  if (d > h + reach) return 2;
+  if (d < -(h + reach)) return 1;
+  return (d <= h - reach && d >= reach - h) ? 0 : 3;
+}
+
+// The three axes' classes packed 2 bits each (x lowest), or kImageMixed
+// when some axis is 3.
+constexpr int kImageMixed = 1 << 6;
+__device__ __forceinline__ int image_code(double dx, double dy, double dz,
+                                          const double L[3], double reach) {
+  const int cx = image_class(dx, 0.5 * L[0], reach);
+  const int cy = image_class(dy, 0.5 * L[1], reach);
+  const int cz = image_class(dz, 0.5 * L[2], reach);
+  return (cx == 3 || cy == 3 || cz == 3) ? kImageMixed : (cx | cy << 2 | cz << 4);
+}
+
+// The shift image_class stands for (applied to the far molecule's sites, so
+// a site pair's displacement differs from wrap1's only in the rounding of
+// coordinates that are shifted by L).
+__device__ __forceinline__ double image_shift(int c, double L) {
+  return c == 1 ? L : (c == 2 ? -L : 0.0);
 }
 
 // 1/sqrt(x) in FP64: the hardware estimate (v_rsq_f64) refined by two

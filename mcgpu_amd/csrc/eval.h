@@ -22,31 +22,45 @@
 
 namespace mcgpu {
 
+// Marks a staged site (in EvalShared::sp) of a hit whose site pairs may
+// wrap differently: its pairs take wrap1 per axis.
+constexpr int kPairWrap = 1 << 20;
+
 template <int NT>
 struct EvalShared {
   static constexpr int kCap = NT * 2 < 256 ? 256 : NT * 2;
   static constexpr int kItemCap = 2048;  // items with a cell entry in item_cell
-  static constexpr int kSiteCap = NT >= 256 ? 768 : 192;  // staged hit sites
+  // staged hit sites; 1024-lane groups keep their LDS under 64 KiB, which
+  // LDS instructions address from one base with an immediate offset
+  static constexpr int kSiteCap = NT >= 1024 ? 768 : (NT >= 512 ? 1536 : (NT >= 256 ? 768 : 192));
   static constexpr int kNW = NT / 64;
-  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
-  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
-  SiteParam64 q64[kMaxSites];
-  SiteParam32 q32[kMaxSites];
-  int32_t pofs[kMaxSites];  // primary offsets within the molecule
-  // in-range molecules of the current evaluation (hits), compacted:
-  int32_t cand_start[kCap];  // first site
-  int32_t cand_param[kCap];  // first site-parameter entry
-  int32_t cand_lf[kCap];     // site count | flags << 8 (bit0 old, bit1 new)
-  int32_t cell_id[64];       // cells of the current range (at most 4x4x4)
-  int32_t cell_pre[65];      // prefix of their occupancies
-  uint8_t item_cell[kItemCap];  // range cell of each candidate item
-  // sites of a chunk of hits, staged for the pair sums: coordinates and the
-  // site-parameter entry (-1: no such site in that hit)
+  // What the pair loop reads comes first, within the 64 KiB that LDS
+  // instructions reach with an immediate offset from one base.
+  // Sites of a chunk of hits, staged for the pair sums: coordinates and the
+  // site-parameter entry (-1: no such site in that hit); coordinates are
+  // shifted by the periodic image of their pairs with the moving molecule,
+  // except for hits whose site pairs may wrap differently (kPairWrap in sp
+  // marks those).
   double sx[kSiteCap], sy[kSiteCap], sz[kSiteCap];
   int32_t sp[kSiteCap];
   // the box's site-parameter table
   SiteParam64 prm64[kParamCap];
   SiteParam32 prm32[kParamCap];
+  // in-range molecules of the current evaluation (hits), compacted:
+  // site count | flags << 8 (bit0 old, bit1 new) | image code << 10 (see
+  // image_code; the same for the old and the proposed position, else
+  // kImageMixed)
+  int32_t cand_lf[kCap];
+  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
+  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
+  SiteParam64 q64[kMaxSites];
+  SiteParam32 q32[kMaxSites];
+  int32_t pofs[kMaxSites];   // primary offsets within the molecule
+  int32_t cand_start[kCap];  // first site
+  int32_t cand_param[kCap];  // first site-parameter entry
+  int32_t cell_id[64];       // cells of the current range (at most 4x4x4)
+  int32_t cell_pre[65];      // prefix of their occupancies
+  uint8_t item_cell[kItemCap];  // range cell of each candidate item
   double red[kNW];
   double red2[kNW];
   unsigned long long redc[kNW];  // sites << 32 | candidates, per wave
@@ -168,10 +182,12 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
       int pi = -1;
       if (b < (lf & 0xff)) {
         const int site = sh.cand_start[k] + b;
-        sh.sx[u] = bv.x[site];
-        sh.sy[u] = bv.y[site];
-        sh.sz[u] = bv.z[site];
-        pi = sh.cand_param[k] + b;
+        const int img = (lf >> 10) & 0x7f;
+        const bool mixed = img & kImageMixed;
+        sh.sx[u] = bv.x[site] + (mixed ? 0.0 : image_shift(img & 3, L0));
+        sh.sy[u] = bv.y[site] + (mixed ? 0.0 : image_shift((img >> 2) & 3, L1));
+        sh.sz[u] = bv.z[site] + (mixed ? 0.0 : image_shift((img >> 4) & 3, L2));
+        pi = (sh.cand_param[k] + b) | (mixed ? kPairWrap : 0);
       }
       sh.sp[u] = pi;
     }
@@ -200,29 +216,39 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
 #pragma unroll
       for (int v = 0; v < U; v++) {
         if (pi[v] < 0) continue;
+        const int pk = pi[v] & (kPairWrap - 1);
+        // displacements: the staged site carries its hit's periodic image,
+        // or wrap1 per site pair when the hit's site pairs may wrap
+        // differently
+        double dxo = xb[v] - xo, dyo = yb[v] - yo, dzo = zb[v] - zo;
+        double dxn = xb[v] - xn, dyn = yb[v] - yn, dzn = zb[v] - zn;
+        if (pi[v] & kPairWrap) {
+          dxo = wrap1(dxo, L0);
+          dyo = wrap1(dyo, L1);
+          dzo = wrap1(dzo, L2);
+          dxn = wrap1(dxn, L0);
+          dyn = wrap1(dyn, L1);
+          dzn = wrap1(dzn, L2);
+        }
         if (MIXED) {
-          const SiteParam32 pb = sh.prm32[pi[v]];
-          const SiteParam64& pd = sh.prm64[pi[v]];
+          const SiteParam32 pb = sh.prm32[pk];
+          const SiteParam64& pd = sh.prm64[pk];
           if (f[v] & 1) {
-            e_old += pair_energy32(wrap1(xb[v] - xo, L0), wrap1(yb[v] - yo, L1),
-                                   wrap1(zb[v] - zo, L2), sa32, ea32, qa32, pb.s, pb.e, pb.q,
+            e_old += pair_energy32(dxo, dyo, dzo, sa32, ea32, qa32, pb.s, pb.e, pb.q,
                                    sa, ea, pd.s, pd.e);
             npairs++;
           }
           if (f[v] & 2)
-            e_new += pair_energy32(wrap1(xb[v] - xn, L0), wrap1(yb[v] - yn, L1),
-                                   wrap1(zb[v] - zn, L2), sa32, ea32, qa32, pb.s, pb.e, pb.q,
+            e_new += pair_energy32(dxn, dyn, dzn, sa32, ea32, qa32, pb.s, pb.e, pb.q,
                                    sa, ea, pd.s, pd.e);
         } else {
-          const SiteParam64 pb = sh.prm64[pi[v]];
+          const SiteParam64 pb = sh.prm64[pk];
           if (f[v] & 1) {
-            e_old += pair_energy64(wrap1(xb[v] - xo, L0), wrap1(yb[v] - yo, L1),
-                                   wrap1(zb[v] - zo, L2), sa, ea, qa, pb.s, pb.e, pb.q);
+            e_old += pair_energy64(dxo, dyo, dzo, sa, ea, qa, pb.s, pb.e, pb.q);
             npairs++;
           }
           if (f[v] & 2)
-            e_new += pair_energy64(wrap1(xb[v] - xn, L0), wrap1(yb[v] - yn, L1),
-                                   wrap1(zb[v] - zn, L2), sa, ea, qa, pb.s, pb.e, pb.q);
+            e_new += pair_energy64(dxn, dyn, dzn, sa, ea, qa, pb.s, pb.e, pb.q);
         }
       }
     }
@@ -342,6 +368,16 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
       f = primary_in_range(bv, ps, pc, sh.ox, sh.oy, sh.oz, sh.pofs, npri) ? 1 : 0;
       if (has_new)
         f |= primary_in_range(bv, ps, pc, sh.nx, sh.ny, sh.nz, sh.pofs, npri) ? 2 : 0;
+      if (f) {
+        // periodic image of the pair, from the first primaries
+        const int o = sh.pofs[0];
+        const double qx = bv.px[ps], qy = bv.py[ps], qz = bv.pz[ps];
+        int img = image_code(qx - sh.ox[o], qy - sh.oy[o], qz - sh.oz[o], bv.L, bv.reach2);
+        if (has_new &&
+            image_code(qx - sh.nx[o], qy - sh.ny[o], qz - sh.nz[o], bv.L, bv.reach2) != img)
+          img = kImageMixed;
+        f |= img << 2;
+      }
     }
     // compaction: rank within the wave, wave counts through LDS
     const unsigned long long bal = __ballot(f != 0);

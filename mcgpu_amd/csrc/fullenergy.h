@@ -25,6 +25,7 @@ struct WaveEval {
   int32_t hit_start[kWaveHits];
   int32_t hit_param[kWaveHits];
   int32_t hit_len[kWaveHits];
+  int32_t hit_img[kWaveHits];  // periodic image of the pair (image_code)
 };
 
 template <bool MIXED>
@@ -44,14 +45,23 @@ __device__ __forceinline__ double wave_hits_energy(const BoxView& bv, const Wave
     const int pb = W.hit_param[k] + b;
     const SiteParam64 B = P64[pb];
     if (B.real == 0.0) continue;
-    const double xb = bv.x[sb], yb = bv.y[sb], zb = bv.z[sb];
+    // site b carries the pair's periodic image, unless the hit's site pairs
+    // may wrap differently (then wrap1 per site pair)
+    const int img = W.hit_img[k];
+    const bool per_pair = img & kImageMixed;
+    const double xb = bv.x[sb] + (per_pair ? 0.0 : image_shift(img & 3, L0));
+    const double yb = bv.y[sb] + (per_pair ? 0.0 : image_shift((img >> 2) & 3, L1));
+    const double zb = bv.z[sb] + (per_pair ? 0.0 : image_shift((img >> 4) & 3, L2));
     for (int a = 0; a < len_i; a++) {
       const int pa = W.pa[a];
       const SiteParam64 A = P64[pa];
       if (A.real == 0.0) continue;
-      const double dx = wrap1(xb - W.xa[a], L0);
-      const double dy = wrap1(yb - W.ya[a], L1);
-      const double dz = wrap1(zb - W.za[a], L2);
+      double dx = xb - W.xa[a], dy = yb - W.ya[a], dz = zb - W.za[a];
+      if (per_pair) {
+        dx = wrap1(dx, L0);
+        dy = wrap1(dy, L1);
+        dz = wrap1(dz, L2);
+      }
       if (MIXED) {
         const SiteParam32 A32 = P32[pa], B32 = P32[pb];
         e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e, B.s,
@@ -143,7 +153,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
       j = i + 1 + item;
     }
     bool hit = false;
-    int hs = 0, hp = 0, hl = 0;
+    int hs = 0, hp = 0, hl = 0, himg = 0;
     if (j > i) {
       const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
       const int pcj = bv.single_primary ? 1 : bv.mol_pidx_count[j];
@@ -157,6 +167,8 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
         }
       }
       if (hit) {
+        himg = image_code(bv.px[psj] - readlane_f64(pix, 0), bv.py[psj] - readlane_f64(piy, 0),
+                          bv.pz[psj] - readlane_f64(piz, 0), bv.L, bv.reach2);
         if (bv.uni_len > 0) {
           hl = bv.uni_len;
           hs = j * hl;
@@ -181,6 +193,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
       W.hit_start[pos] = hs;
       W.hit_param[pos] = hp;
       W.hit_len[pos] = hl;
+      W.hit_img[pos] = himg;
     }
     nh += cnt;
     __builtin_amdgcn_wave_barrier();
