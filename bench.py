@@ -192,11 +192,13 @@ def cpu_baseline(jobs, pool, budget_s: float):
 
 
 # VALU issue slots per site pair in the full-energy kernel's inner loop
-# (FP64 path, read from this build's gfx950 code object: 49 full-rate VALU
-# instructions -- 15 of them the three single wraps, 8 the rsq refinement --
-# plus v_rsq_f64 at quarter rate = 4 slots); with 256 CUs x 4 SIMDs x 16
-# lanes per cycle at 2.4 GHz this sets the pair-interaction roof.
-VALU_PER_PAIR_FP64 = 53
+# (FP64 path without the per-pair wrap, read from this build's gfx950 code
+# object: 29 full-rate VALU instructions -- displacement 3, r^2 3, the rsq
+# refinement 7, LJ and Coulomb 10, the r^2 == 0 guard 3, the sum 1, two
+# address steps -- plus v_rsq_f64 at quarter rate = 4 slots); with 256 CUs
+# x 4 SIMDs x 16 lanes per cycle at 2.4 GHz this sets the pair-interaction
+# roof.
+VALU_PER_PAIR_FP64 = 33
 VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
 
 

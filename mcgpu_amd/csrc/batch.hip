@@ -520,13 +520,17 @@ __global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k
   c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
 }
 
+// One instance per precision: each takes the listed boxes of its precision
+// (the other instance's blocks return at once).
+template <bool MIXED>
 __global__ __launch_bounds__(256) void k_energy_waves(const BatchBox* table,
                                                       const VolItem* items) {
   __shared__ WaveEval ws[4];
   __shared__ SiteParam64 p64[kParamCap];
   __shared__ SiteParam32 p32[kParamCap];
   const BatchBox& B = table[items[blockIdx.y].slot];
-  full_energy_block(B.bv, B.precision == MCGPU_MIXED, B.partial, nullptr, ws, p64, p32);
+  if ((B.precision == MCGPU_MIXED) != MIXED) return;
+  full_energy_block<MIXED>(B.bv, B.partial, nullptr, ws, p64, p32);
 }
 
 // Fixed-order sum of entry blockIdx.x's per-molecule partials into
@@ -921,8 +925,9 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
 // energies into d_volout[2 i + which]; cell lists for the current geometry.
 int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
                        int which, hipStream_t st) {
-  hipLaunchKernelGGL(k_energy_waves, dim3(std::min((max_mol + 3) / 4, 2048), k), dim3(256), 0,
-                     st, b->d_table, items);
+  const dim3 grid(std::min((max_mol + 3) / 4, 2048), k);
+  hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
+  hipLaunchKernelGGL(k_energy_waves<true>, grid, dim3(256), 0, st, b->d_table, items);
   hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, st, b->d_table, items, which,
                      b->d_volout);
   MCGPU_HIP_TRY(hipGetLastError());
@@ -1395,8 +1400,12 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
     return rc;
   }
   // start energy: the full-energy pass over this one box
-  hipLaunchKernelGGL(k_energy_waves, dim3(std::min((v.n_mol + 3) / 4, 2048), 1), dim3(256), 0,
-                     st, b->d_table, b->d_one + s);
+  if (p->precision == MCGPU_MIXED)
+    hipLaunchKernelGGL(k_energy_waves<true>, dim3(std::min((v.n_mol + 3) / 4, 2048), 1),
+                       dim3(256), 0, st, b->d_table, b->d_one + s);
+  else
+    hipLaunchKernelGGL(k_energy_waves<false>, dim3(std::min((v.n_mol + 3) / 4, 2048), 1),
+                       dim3(256), 0, st, b->d_table, b->d_one + s);
   hipLaunchKernelGGL(k_reduce_multi, dim3(1), dim3(256), 0, st, b->d_table, b->d_one + s, 0,
                      E.total + 2);
   MCGPU_HIP_TRY(hipGetLastError());

@@ -149,12 +149,13 @@ __global__ __launch_bounds__(NT) void k_verlet_build(BoxView bv, double rv2) {
 
 // Full-system energy of the box (SimulationStep.cpp:40-71), one wave per
 // molecule (see fullenergy.h): partial[i] = E_i(i), pairs[i] = site pairs.
-__global__ __launch_bounds__(256) void k_system_waves(BoxView bv, int mixed, double* partial,
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_system_waves(BoxView bv, double* partial,
                                                       double* pairs) {
   __shared__ WaveEval ws[4];
   __shared__ SiteParam64 p64[kParamCap];
   __shared__ SiteParam32 p32[kParamCap];
-  full_energy_block(bv, mixed != 0, partial, pairs, ws, p64, p32);
+  full_energy_block<MIXED>(bv, partial, pairs, ws, p64, p32);
 }
 
 // Fixed-order sum of n partials into out[0] (and n counts into out[1]).
@@ -852,8 +853,12 @@ extern "C" int mcgpu_system_energy(mcgpu_box* b, double* energy) {
   const int n = b->bv.n_mol;
   const int grid = std::min((n + 3) / 4, 2048);
   if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
-  hipLaunchKernelGGL(k_system_waves, dim3(grid), dim3(256), 0, b->stream, b->bv,
-                     (int)(b->precision == MCGPU_MIXED), b->d_partial, b->d_pairs);
+  if (b->precision == MCGPU_MIXED)
+    hipLaunchKernelGGL(k_system_waves<true>, dim3(grid), dim3(256), 0, b->stream, b->bv,
+                       b->d_partial, b->d_pairs);
+  else
+    hipLaunchKernelGGL(k_system_waves<false>, dim3(grid), dim3(256), 0, b->stream, b->bv,
+                       b->d_partial, b->d_pairs);
   MCGPU_HIP_TRY(hipGetLastError());
   if (b->ev1) MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, b->stream, b->d_partial,

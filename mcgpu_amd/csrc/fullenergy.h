@@ -9,73 +9,140 @@
 
 namespace mcgpu {
 
-// Full-system energy (SimulationStep.cpp:40-71: the sum over i of E_i(i),
-// partners j > i) of every listed box, one wave per molecule i: the lanes
-// scan the candidates (the cells around i's first primary, or all j > i),
-// test the primary cutoff (SimulationStep.cpp:317-330), compact the hits
-// into the wave's LDS list by ballot, then sum the site pairs of the hits
-// (each lane one (hit, site b) item against every site a of i), and the
-// wave sum goes to partial[i].  Waves never wait for each other: a
-// throughput kernel that keeps many waves per CU in flight.
-constexpr int kWaveHits = 256;
+// Full-system energy (SimulationStep.cpp:40-71: every molecule pair within
+// the primary cutoff of SimulationStep.cpp:317-330 once), one wave per
+// molecule i and no barriers between waves:
+//  * the lanes scan i's candidate partners -- with cell lists, i's own cell
+//    (partners j > i) and the 13 neighbour cells on one side of it (any j),
+//    so that every pair of cells is visited from one end only; without, all
+//    j > i -- and compact the hits into the wave's LDS list by ballot;
+//  * the hits' real sites are staged densely in LDS (coordinates already
+//    carrying the pair's periodic image), and each lane takes one staged
+//    site b against every real site a of i, whose coordinates and
+//    parameters the wave holds in LDS.
+// partial[i] receives i's share; the shares are reduced in a fixed order
+// elsewhere, so the total is deterministic.  Which molecule of a pair holds
+// its terms differs from the reference's E_i(i) split (j > i); the set of
+// pairs, and so the total up to summation order, is the same.
+constexpr int kWaveHits = 128;   // hits buffered before their pairs are summed
+constexpr int kWaveItems = 128;  // staged sites b per pass of the pair loop
 
 struct WaveEval {
-  double xa[kMaxSites], ya[kMaxSites], za[kMaxSites];  // sites of molecule i
-  int32_t pa[kMaxSites];                               // their parameter entries
+  // real sites of molecule i (compacted): coordinates, FP64 and FP32 pair
+  // parameters
+  double ax[kMaxSites], ay[kMaxSites], az[kMaxSites];
+  SiteParam64 ap[kMaxSites];
+  SiteParam32 ap32[kMaxSites];
   int32_t hit_start[kWaveHits];
   int32_t hit_param[kWaveHits];
   int32_t hit_len[kWaveHits];
   int32_t hit_img[kWaveHits];  // periodic image of the pair (image_code)
+  // staged real sites b of the hits: coordinates and parameter entry, with
+  // kPairWrap set when the hit's site pairs may wrap differently
+  double bx[kWaveItems], by[kWaveItems], bz[kWaveItems];
+  int32_t bp[kWaveItems];
 };
 
+// Pairs of the staged sites b [0, n) with the na real sites of i.
 template <bool MIXED>
-__device__ __forceinline__ double wave_hits_energy(const BoxView& bv, const WaveEval& W,
-                                                   int len_i, int nh,
-                                                   const SiteParam64* P64,
-                                                   const SiteParam32* P32, unsigned& npairs) {
-  const int M = bv.max_len;
-  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
+__device__ __forceinline__ double wave_items_energy(const BoxView& bv, const WaveEval& W,
+                                                    int na, int n, const SiteParam64* P64,
+                                                    const SiteParam32* P32) {
   const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
   double e = 0.0;
-  for (int u = wlane(); u < nh * M; u += 64) {
-    const int k = div_small(u, magic);
-    const int b = u - k * M;
-    if (b >= W.hit_len[k]) continue;
-    const int sb = W.hit_start[k] + b;
-    const int pb = W.hit_param[k] + b;
+  for (int it = wlane(); it < n; it += 64) {
+    const double xb = W.bx[it], yb = W.by[it], zb = W.bz[it];
+    const int bp = W.bp[it];
+    const int pb = bp & (kPairWrap - 1);
     const SiteParam64 B = P64[pb];
-    if (B.real == 0.0) continue;
-    // site b carries the pair's periodic image, unless the hit's site pairs
-    // may wrap differently (then wrap1 per site pair)
-    const int img = W.hit_img[k];
-    const bool per_pair = img & kImageMixed;
-    const double xb = bv.x[sb] + (per_pair ? 0.0 : image_shift(img & 3, L0));
-    const double yb = bv.y[sb] + (per_pair ? 0.0 : image_shift((img >> 2) & 3, L1));
-    const double zb = bv.z[sb] + (per_pair ? 0.0 : image_shift((img >> 4) & 3, L2));
-    for (int a = 0; a < len_i; a++) {
-      const int pa = W.pa[a];
-      const SiteParam64 A = P64[pa];
-      if (A.real == 0.0) continue;
-      double dx = xb - W.xa[a], dy = yb - W.ya[a], dz = zb - W.za[a];
-      if (per_pair) {
-        dx = wrap1(dx, L0);
-        dy = wrap1(dy, L1);
-        dz = wrap1(dz, L2);
+    const SiteParam32 B32 = P32[pb];
+    if (bp & kPairWrap) {
+      for (int a = 0; a < na; a++) {
+        const double dx = wrap1(xb - W.ax[a], L0);
+        const double dy = wrap1(yb - W.ay[a], L1);
+        const double dz = wrap1(zb - W.az[a], L2);
+        const SiteParam64& A = W.ap[a];
+        if (MIXED) {
+          const SiteParam32& A32 = W.ap32[a];
+          e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e,
+                             B.s, B.e);
+        } else {
+          e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
+        }
       }
-      if (MIXED) {
-        const SiteParam32 A32 = P32[pa], B32 = P32[pb];
-        e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e, B.s,
-                           B.e);
-      } else {
-        e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
+    } else {
+      for (int a = 0; a < na; a++) {
+        const double dx = xb - W.ax[a], dy = yb - W.ay[a], dz = zb - W.az[a];
+        const SiteParam64& A = W.ap[a];
+        if (MIXED) {
+          const SiteParam32& A32 = W.ap32[a];
+          e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e,
+                             B.s, B.e);
+        } else {
+          e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
+        }
       }
-      npairs++;
     }
   }
   return e;
 }
 
-// Returns E_i(i) (wave-uniform); pairs receives the real site pairs summed.
+// Sums the pairs of hits [0, nh) with i: their real sites are staged
+// kWaveItems at a time (whole hits per pass of 64 slots, compacted by
+// ballot), each pass followed by the pair loop.  npairs counts the site
+// pairs.
+template <bool MIXED>
+__device__ __forceinline__ double wave_hits_energy(const BoxView& bv, WaveEval& W, int na,
+                                                   int nh, const SiteParam64* P64,
+                                                   const SiteParam32* P32, unsigned& npairs) {
+  const int M = bv.max_len;
+  const int per_pass = 64 / M;  // whole hits per staging pass
+  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
+  const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
+  const int lane = wlane();
+  double e = 0.0;
+  int n = 0;
+  for (int k0 = 0; k0 < nh; k0 += per_pass) {
+    if (n + 64 > kWaveItems) {  // wave-uniform
+      __builtin_amdgcn_wave_barrier();
+      e += wave_items_energy<MIXED>(bv, W, na, n, P64, P32);
+      npairs += (unsigned)(n * na);
+      __builtin_amdgcn_wave_barrier();
+      n = 0;
+    }
+    const int kk = div_small(lane, magic);
+    const int b = lane - kk * M;
+    const int k = k0 + kk;
+    bool ok = false;
+    int pb = 0, site = 0, img = 0;
+    if (kk < per_pass && k < nh && b < W.hit_len[k]) {
+      pb = W.hit_param[k] + b;
+      ok = P64[pb].real != 0.0;
+      site = W.hit_start[k] + b;
+      img = W.hit_img[k];
+    }
+    const unsigned long long bal = __ballot(ok);
+    if (ok) {
+      const int pos = n + wave_rank(bal);
+      const bool mixed = img & kImageMixed;
+      W.bx[pos] = bv.x[site] + (mixed ? 0.0 : image_shift(img & 3, L0));
+      W.by[pos] = bv.y[site] + (mixed ? 0.0 : image_shift((img >> 2) & 3, L1));
+      W.bz[pos] = bv.z[site] + (mixed ? 0.0 : image_shift((img >> 4) & 3, L2));
+      W.bp[pos] = pb | (mixed ? kPairWrap : 0);
+    }
+    n += __popcll(bal);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (n > 0) {
+    e += wave_items_energy<MIXED>(bv, W, na, n, P64, P32);
+    npairs += (unsigned)(n * na);
+  }
+  __builtin_amdgcn_wave_barrier();
+  return e;
+}
+
+// Returns molecule i's share of the full energy (wave-uniform); pairs
+// receives the real site pairs summed.
 template <bool MIXED>
 __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEval& W, int i,
                                                        const SiteParam64* P64,
@@ -98,11 +165,21 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     psi = bv.single_primary ? i : m.pstart;
     pci = bv.single_primary ? 1 : bv.mol_pidx_count[i];
   }
-  if (lane < len) {
-    W.xa[lane] = bv.x[start + lane];
-    W.ya[lane] = bv.y[start + lane];
-    W.za[lane] = bv.z[start + lane];
-    W.pa[lane] = param + lane;
+  // i's real sites, compacted
+  int na;
+  {
+    const bool real = lane < len && P64[param + lane].real != 0.0;
+    const unsigned long long bal = __ballot(real);
+    if (real) {
+      const int pos = wave_rank(bal);
+      W.ax[pos] = bv.x[start + lane];
+      W.ay[pos] = bv.y[start + lane];
+      W.az[pos] = bv.z[start + lane];
+      W.ap[pos] = P64[param + lane];
+      W.ap32[pos] = P32[param + lane];
+    }
+    na = __popcll(bal);
+    __builtin_amdgcn_wave_barrier();
   }
   // the primaries of i (at most kMaxSites) as lane values
   double pix = 0.0, piy = 0.0, piz = 0.0;
@@ -111,26 +188,53 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     piy = bv.py[psi + lane];
     piz = bv.pz[psi + lane];
   }
-  __builtin_amdgcn_wave_barrier();
+  const double p0x = readlane_f64(pix, 0), p0y = readlane_f64(piy, 0),
+               p0z = readlane_f64(piz, 0);
   const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
-  // candidate source
+  // candidate source: lane c < ncell holds cell c of the scan and its
+  // occupancy prefix; cell 0 is i's own (partners j > i)
   const CellGrid& g = bv.cells;
   const bool cells = bv.mode == MCGPU_NB_CELL && g.overflow[0] == 0;
-  CellRange r{};
-  int ncell = 0, incl = 0, nitems = bv.n_mol - (i + 1);
+  int ncell = 0, incl = 0, nitems = bv.n_mol - (i + 1), own_full = 0;
+  int my_cell = 0;
   if (cells) {
     int co[3];
-    co[0] = cell_coord(readlane_f64(pix, 0), g.inv_w[0], g.n[0]);
-    co[1] = cell_coord(readlane_f64(piy, 0), g.inv_w[1], g.n[1]);
-    co[2] = cell_coord(readlane_f64(piz, 0), g.inv_w[2], g.n[2]);
-    r = make_range(g, co, co);  // at most 3 x 3 x 3 cells
-    ncell = r.total();
-    int n = 0;
-    if (lane < ncell) {
-      n = g.count[r.cell(g, lane)];
-      n = n < g.cap ? n : g.cap;
+    co[0] = cell_coord(p0x, g.inv_w[0], g.n[0]);
+    co[1] = cell_coord(p0y, g.inv_w[1], g.n[1]);
+    co[2] = cell_coord(p0z, g.inv_w[2], g.n[2]);
+    if (g.n[0] >= 3 && g.n[1] >= 3 && g.n[2] >= 3) {
+      // own cell + the 13 offsets (dx, dy, dz) > (0, 0, 0) lexicographically
+      ncell = 14;
+      if (lane < 14) {
+        int d0 = 0, d1 = 0, d2 = 0;
+        if (lane > 0) {
+          const int t = lane + 13;  // 14..26 -> offsets above the centre
+          d0 = t / 9 - 1;
+          d1 = (t / 3) % 3 - 1;
+          d2 = t % 3 - 1;
+        }
+        int a = co[0] + d0, b = co[1] + d1, c = co[2] + d2;
+        a += a < 0 ? g.n[0] : 0;
+        a -= a >= g.n[0] ? g.n[0] : 0;
+        b += b < 0 ? g.n[1] : 0;
+        b -= b >= g.n[1] ? g.n[1] : 0;
+        c += c < 0 ? g.n[2] : 0;
+        c -= c >= g.n[2] ? g.n[2] : 0;
+        my_cell = cell_index(g, a, b, c);
+      }
+    } else {
+      // fewer than 3 cells along some axis: every cell around i, j > i
+      const CellRange r = make_range(g, co, co);
+      ncell = r.total();
+      own_full = 1;
+      if (lane < ncell) my_cell = r.cell(g, lane);
     }
-    incl = wave_incl_scan_i32(n);
+    int cnt = 0;
+    if (lane < ncell) {
+      cnt = g.count[my_cell];
+      cnt = cnt < g.cap ? cnt : g.cap;
+    }
+    incl = wave_incl_scan_i32(cnt);
     nitems = readlane_i32(incl, 63);
   }
   double e = 0.0;
@@ -138,6 +242,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
   for (int base = 0; base < nitems; base += 64) {
     const int item = base + lane;
     int j = -1;
+    bool need_greater = true;
     if (cells) {
       // cell holding `item`: the number of cells whose inclusive prefix is
       // <= item (all lanes take part in the lane reads)
@@ -148,13 +253,15 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
         if (probe < ncell && v <= item) lo += step;
       }
       const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
-      if (item < nitems) j = g.slots[r.cell(g, lo) * g.cap + (item - (lo > 0 ? before : 0))];
+      const int cell = __shfl(my_cell, lo < 63 ? lo : 63);
+      if (item < nitems) j = g.slots[cell * g.cap + (item - (lo > 0 ? before : 0))];
+      need_greater = own_full || lo == 0;
     } else if (item < nitems) {
       j = i + 1 + item;
     }
     bool hit = false;
     int hs = 0, hp = 0, hl = 0, himg = 0;
-    if (j > i) {
+    if (j >= 0 && j != i && (!need_greater || j > i)) {
       const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
       const int pcj = bv.single_primary ? 1 : bv.mol_pidx_count[j];
       for (int q = 0; q < pcj; q++) {
@@ -167,8 +274,8 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
         }
       }
       if (hit) {
-        himg = image_code(bv.px[psj] - readlane_f64(pix, 0), bv.py[psj] - readlane_f64(piy, 0),
-                          bv.pz[psj] - readlane_f64(piz, 0), bv.L, bv.reach2);
+        himg = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
+                          bv.reach2);
         if (bv.uni_len > 0) {
           hl = bv.uni_len;
           hs = j * hl;
@@ -184,8 +291,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     const unsigned long long bal = __ballot(hit);
     const int cnt = __popcll(bal);
     if (nh + cnt > kWaveHits) {  // wave-uniform
-      e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32, np);
-      __builtin_amdgcn_wave_barrier();
+      e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np);
       nh = 0;
     }
     if (hit) {
@@ -198,19 +304,18 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     nh += cnt;
     __builtin_amdgcn_wave_barrier();
   }
-  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, len, nh, P64, P32, np);
-  __builtin_amdgcn_wave_barrier();
-  pairs = wave_sum_u32(np);
+  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np);
+  pairs = np;  // wave-uniform count
   return wave_sum_f64(e);
 }
 
 // A 256-lane work-group's share of a box's full energy: molecules
-// blockIdx.x * 4 + wave, + 4 gridDim.x, ...; partial[i] = E_i(i) and, when
-// pairs is given, pairs[i] = the real site pairs summed.
-__device__ __forceinline__ void full_energy_block(const BoxView& bv, bool mixed,
-                                                  double* partial, double* pairs,
-                                                  WaveEval* ws, SiteParam64* p64,
-                                                  SiteParam32* p32) {
+// blockIdx.x * 4 + wave, + 4 gridDim.x, ...; partial[i] = i's share and,
+// when pairs is given, pairs[i] = the real site pairs it summed.
+template <bool MIXED>
+__device__ __forceinline__ void full_energy_block(const BoxView& bv, double* partial,
+                                                  double* pairs, WaveEval* ws,
+                                                  SiteParam64* p64, SiteParam32* p32) {
   for (int k = threadIdx.x; k < bv.n_param; k += 256) {
     p64[k] = bv.p64[k];
     p32[k] = bv.p32[k];
@@ -219,8 +324,7 @@ __device__ __forceinline__ void full_energy_block(const BoxView& bv, bool mixed,
   const int w = threadIdx.x >> 6;
   for (int i = blockIdx.x * 4 + w; i < bv.n_mol; i += gridDim.x * 4) {
     unsigned np = 0;
-    const double e = mixed ? wave_molecule_energy<true>(bv, ws[w], i, p64, p32, np)
-                           : wave_molecule_energy<false>(bv, ws[w], i, p64, p32, np);
+    const double e = wave_molecule_energy<MIXED>(bv, ws[w], i, p64, p32, np);
     if (wlane() == 0) {
       partial[i] = e;
       if (pairs) pairs[i] = (double)np;
