@@ -174,3 +174,45 @@ def test_dummy_sites_and_errors():
         db.molecule_energy(256)
     with pytest.raises(m.McgpuError):
         db.molecule_energy(0, 257)
+
+
+def test_pairs_near_half_box_take_the_per_pair_wrap():
+    """A box whose molecules span a good part of L/2: the pair kernels use
+    one periodic image per molecule pair only where no site pair can wrap
+    differently, and the per-site-pair wrap elsewhere.  Every molecule
+    energy, the system energy and a move trace must match the oracle (whose
+    wrap is per site pair throughout)."""
+    hb = m.HostBox.generate(m.BICYCLIC, 40, edge=22.0, cutoff=11.0, seed=17, mc_seed=5)
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    for i in range(40):
+        assert db.molecule_energy(i) == pytest.approx(ob.molecule_energy(i), rel=1e-12, abs=1e-9)
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=1e-9)
+    steps = 1500
+    ob2 = O.OracleBox(hb.arrays())
+    _, _, _, want = ob2.run_nvt(hb.mc_seed, steps, use_libc=False)
+    res, tr = m.plain_run(hb, steps, precision=m.FP64)
+    assert np.array_equal(tr.mol, want["mol"])
+    assert np.array_equal(tr.accepted, want["accepted"])
+    assert de_ok(tr.de, want)
+
+
+def test_reshaped_molecule_keeps_energies_exact():
+    """Coordinates handed in through the ABI may change a molecule's shape
+    (here one molecule is stretched to twice its size): the kernels' bound on
+    how far sites sit from their first primary must follow."""
+    hb = m.workload("water256")
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    xyz = db.get_molecule(7).reshape(3, -1)
+    p = xyz[:, :1]
+    stretched = (p + 2.0 * (xyz - p)).reshape(-1)
+    db.set_molecule(7, stretched)
+    a = hb.arrays()
+    s = int(a["mol_start"][7])
+    for k in range(3):
+        a["xyz"[k]][s:s + 3] = stretched[k * 3:(k + 1) * 3]
+    ob = O.OracleBox(a)
+    for i in (7, 0, 100, 255):
+        assert db.molecule_energy(i) == pytest.approx(ob.molecule_energy(i), rel=1e-12, abs=1e-9)
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=1e-9)
