@@ -163,10 +163,34 @@ extern "C" int mcgpu_server_run(mcgpu_server* s) {
       }
       continue;
     }
+    // The tick runs asynchronously: jobs that arrive meanwhile are admitted
+    // (host set-up and queued uploads) while it runs; retirement below
+    // waits for it.
     const double tt = now();
-    int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 1);
+    int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 0);
     if (rc != MCGPU_OK) return rc;
     t_run += now() - tt;
+    {
+      const double t2 = now();
+      while (next < order.size() && s->jobs[order[next]].res.arrival_s <= t2) {
+        Job& j = s->jobs[order[next]];
+        const mcgpu_box_desc d = mcgpu::hostbox_ref(j.box).desc();
+        int32_t slot = -1;
+        rc = mcgpu_batch_add(s->batch, &d, &j.params, &slot);
+        if (rc == MCGPU_ERR_NOMEM) break;  // wait for room after retirement
+        next++;
+        if (rc != MCGPU_OK) {
+          j.res.status = rc;
+          j.res.admit_s = j.res.finish_s = now();
+          completed++;
+          continue;
+        }
+        j.slot = slot;
+        j.res.admit_s = now();
+        resident.push_back((int32_t)order[next - 1]);
+      }
+      t_admit += now() - t2;
+    }
     s->stats.ticks++;
     last_progress = now();
     if (log_every > 0 && now() - tt > log_every) {
