@@ -924,10 +924,12 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
 // The batched passes over the first k entries of a device list: full
 // energies into d_volout[2 i + which]; cell lists for the current geometry.
 int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
-                       int which, hipStream_t st) {
+                       int which, hipStream_t st, bool any_fp64, bool any_mixed) {
   const dim3 grid(std::min((max_mol + 3) / 4, 2048), k);
-  hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
-  hipLaunchKernelGGL(k_energy_waves<true>, grid, dim3(256), 0, st, b->d_table, items);
+  if (any_fp64)
+    hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
+  if (any_mixed)
+    hipLaunchKernelGGL(k_energy_waves<true>, grid, dim3(256), 0, st, b->d_table, items);
   hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, st, b->d_table, items, which,
                      b->d_volout);
   MCGPU_HIP_TRY(hipGetLastError());
@@ -962,9 +964,11 @@ int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
   const int K = (int)pend.size();
   int max_mol = 0, max_cells = 0;
+  bool any_fp64 = false, any_mixed = false;
   for (int k = 0; k < K; k++) {
     const int s = pend[k];
     b->h_items[k] = VolItem{s, 0, 1.0};
+    (b->slots[s].params.precision == MCGPU_MIXED ? any_mixed : any_fp64) = true;
     max_mol = std::max(max_mol, b->h_table[s].bv.n_mol);
     max_cells = std::max(max_cells, b->slots[s].lay.ncell);
     // the decision puts the box back to running (it is below its target)
@@ -980,7 +984,7 @@ int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
   if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st)) ||
-      (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st)))
+      (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st, any_fp64, any_mixed)))
     return rc;
   hipLaunchKernelGGL(k_vol_decide, dim3(tb), dim3(64), 0, st, b->d_table, b->d_ctl,
                      b->d_items, K, (const double*)b->d_volout);
