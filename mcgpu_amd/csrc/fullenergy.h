@@ -33,7 +33,7 @@ struct WaveEval {
   double ax[kMaxSites], ay[kMaxSites], az[kMaxSites];
   SiteParam64 ap[kMaxSites];
   SiteParam32 ap32[kMaxSites];
-  int32_t hit_start[kWaveHits];
+  int32_t hit_start[kWaveHits];  // partner id until wave_resolve_hits
   int32_t hit_param[kWaveHits];
   int32_t hit_len[kWaveHits];
   int32_t hit_img[kWaveHits];  // periodic image of the pair (image_code)
@@ -56,7 +56,9 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
     const int pb = bp & (kPairWrap - 1);
     const SiteParam64 B = P64[pb];
     const SiteParam32 B32 = P32[pb];
+    // two site pairs per step: independent chains for the FP64 latency
     if (bp & kPairWrap) {
+#pragma unroll 2
       for (int a = 0; a < na; a++) {
         const double dx = wrap1(xb - W.ax[a], L0);
         const double dy = wrap1(yb - W.ay[a], L1);
@@ -71,6 +73,7 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
         }
       }
     } else {
+#pragma unroll 2
       for (int a = 0; a < na; a++) {
         const double dx = xb - W.ax[a], dy = yb - W.ay[a], dz = zb - W.az[a];
         const SiteParam64& A = W.ap[a];
@@ -87,6 +90,32 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
   return e;
 }
 
+// The scan leaves partner ids in hit_start[0, nh); one lane per hit turns
+// each into its record (first site, site count, parameter entry) and the
+// pair's periodic image, classified from the two first primaries (p0 is
+// i's).  Doing this once per hit rather than once per scanned candidate
+// keeps the scan loop to loads and the cutoff test.
+__device__ __forceinline__ void wave_resolve_hits(const BoxView& bv, WaveEval& W, int nh,
+                                                  double p0x, double p0y, double p0z) {
+  for (int h = wlane(); h < nh; h += 64) {
+    const int j = W.hit_start[h];
+    const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
+    W.hit_img[h] = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
+                              bv.reach2);
+    if (bv.uni_len > 0) {
+      W.hit_start[h] = j * bv.uni_len;
+      W.hit_len[h] = bv.uni_len;
+      W.hit_param[h] = bv.uni_param;
+    } else {
+      const MolMeta m = bv.meta[j];
+      W.hit_start[h] = m.start;
+      W.hit_len[h] = m.len;
+      W.hit_param[h] = m.param;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Sums the pairs of hits [0, nh) with i: their real sites are staged
 // kWaveItems at a time (whole hits per pass of 64 slots, compacted by
 // ballot), each pass followed by the pair loop.  npairs counts the site
@@ -94,7 +123,9 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
 template <bool MIXED>
 __device__ __forceinline__ double wave_hits_energy(const BoxView& bv, WaveEval& W, int na,
                                                    int nh, const SiteParam64* P64,
-                                                   const SiteParam32* P32, unsigned& npairs) {
+                                                   const SiteParam32* P32, unsigned& npairs,
+                                                   double p0x, double p0y, double p0z) {
+  wave_resolve_hits(bv, W, nh, p0x, p0y, p0z);
   const int M = bv.max_len;
   const int per_pass = 64 / M;  // whole hits per staging pass
   const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
@@ -237,74 +268,70 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     incl = wave_incl_scan_i32(cnt);
     nitems = readlane_i32(incl, 63);
   }
+  // candidate of scan item `item` (-1 past the end) and whether it must
+  // exceed i: with cells, the cell whose inclusive prefix first exceeds the
+  // item (binary search over the lane-held prefix, all lanes taking part),
+  // then its slot; without, j = i + 1 + item
+  int span = 1;  // largest power of two <= ncell - 1, at least 1
+  while (2 * span <= ncell - 1) span <<= 1;
+  auto candidate = [&](int item, bool& greater) -> int {
+    if (!cells) {
+      greater = false;
+      return item < nitems ? i + 1 + item : -1;
+    }
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      if (step > span) continue;  // wave-uniform
+      const int probe = lo + step - 1;
+      const int v = __shfl(incl, probe < 63 ? probe : 63);
+      if (probe < ncell && v <= item) lo += step;
+    }
+    const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
+    const int cell = __shfl(my_cell, lo < 63 ? lo : 63);
+    greater = own_full || lo == 0;
+    return item < nitems ? g.slots[cell * g.cap + (item - (lo > 0 ? before : 0))] : -1;
+  };
   double e = 0.0;
   int nh = 0;
+  bool greater_next = false;
+  int j_next = nitems > 0 ? candidate(lane, greater_next) : -1;
   for (int base = 0; base < nitems; base += 64) {
-    const int item = base + lane;
-    int j = -1;
-    bool need_greater = true;
-    if (cells) {
-      // cell holding `item`: the number of cells whose inclusive prefix is
-      // <= item (all lanes take part in the lane reads)
-      int lo = 0;
-      for (int step = 32; step > 0; step >>= 1) {
-        const int probe = lo + step - 1;
-        const int v = __shfl(incl, probe < 63 ? probe : 63);
-        if (probe < ncell && v <= item) lo += step;
-      }
-      const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
-      const int cell = __shfl(my_cell, lo < 63 ? lo : 63);
-      if (item < nitems) j = g.slots[cell * g.cap + (item - (lo > 0 ? before : 0))];
-      need_greater = own_full || lo == 0;
-    } else if (item < nitems) {
-      j = i + 1 + item;
-    }
+    const int j = j_next;
+    const bool need_greater = greater_next;
+    if (base + 64 < nitems) j_next = candidate(base + 64 + lane, greater_next);
     bool hit = false;
-    int hs = 0, hp = 0, hl = 0, himg = 0;
     if (j >= 0 && j != i && (!need_greater || j > i)) {
-      const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
-      const int pcj = bv.single_primary ? 1 : bv.mol_pidx_count[j];
-      for (int q = 0; q < pcj; q++) {
-        const double qx = bv.px[psj + q], qy = bv.py[psj + q], qz = bv.pz[psj + q];
-        for (int p = 0; p < pci; p++) {
-          const double dx = wrap1(qx - readlane_f64(pix, p), L0);
-          const double dy = wrap1(qy - readlane_f64(piy, p), L1);
-          const double dz = wrap1(qz - readlane_f64(piz, p), L2);
-          hit |= dx * dx + dy * dy + dz * dz <= bv.cutoff2;
-        }
-      }
-      if (hit) {
-        himg = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
-                          bv.reach2);
-        if (bv.uni_len > 0) {
-          hl = bv.uni_len;
-          hs = j * hl;
-          hp = bv.uni_param;
-        } else {
-          const MolMeta m = bv.meta[j];
-          hs = m.start;
-          hl = m.len;
-          hp = m.param;
+      if (bv.single_primary) {
+        const double dx = wrap1(bv.px[j] - p0x, L0);
+        const double dy = wrap1(bv.py[j] - p0y, L1);
+        const double dz = wrap1(bv.pz[j] - p0z, L2);
+        hit = dx * dx + dy * dy + dz * dz <= bv.cutoff2;
+      } else {
+        const int psj = bv.mol_pidx_start[j];
+        const int pcj = bv.mol_pidx_count[j];
+        for (int q = 0; q < pcj; q++) {
+          const double qx = bv.px[psj + q], qy = bv.py[psj + q], qz = bv.pz[psj + q];
+          for (int p = 0; p < pci; p++) {
+            const double dx = wrap1(qx - readlane_f64(pix, p), L0);
+            const double dy = wrap1(qy - readlane_f64(piy, p), L1);
+            const double dz = wrap1(qz - readlane_f64(piz, p), L2);
+            hit |= dx * dx + dy * dy + dz * dz <= bv.cutoff2;
+          }
         }
       }
     }
     const unsigned long long bal = __ballot(hit);
     const int cnt = __popcll(bal);
     if (nh + cnt > kWaveHits) {  // wave-uniform
-      e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np);
+      e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np, p0x, p0y, p0z);
       nh = 0;
     }
-    if (hit) {
-      const int pos = nh + wave_rank(bal);
-      W.hit_start[pos] = hs;
-      W.hit_param[pos] = hp;
-      W.hit_len[pos] = hl;
-      W.hit_img[pos] = himg;
-    }
+    if (hit) W.hit_start[nh + wave_rank(bal)] = j;
     nh += cnt;
     __builtin_amdgcn_wave_barrier();
   }
-  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np);
+  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np, p0x, p0y, p0z);
   pairs = np;  // wave-uniform count
   return wave_sum_f64(e);
 }
