@@ -26,6 +26,8 @@ namespace mcgpu {
 // pairs, and so the total up to summation order, is the same.
 constexpr int kWaveHits = 128;   // hits buffered before their pairs are summed
 constexpr int kWaveItems = 128;  // staged sites b per pass of the pair loop
+constexpr int kScanItems = 1024; // scan items mapped per segment (cell lists)
+constexpr int kScanCells = 32;   // cells of one scan (14 half-shell, <= 27 full)
 
 struct WaveEval {
   // real sites of molecule i (compacted): coordinates, FP64 and FP32 pair
@@ -41,6 +43,12 @@ struct WaveEval {
   // kPairWrap set when the hit's site pairs may wrap differently
   double bx[kWaveItems], by[kWaveItems], bz[kWaveItems];
   int32_t bp[kWaveItems];
+  // the scan's non-empty cells in order: cell id (bit 31 set when partners
+  // must exceed i) and first scan item; a bitmap over the current segment
+  // marks the first item of every cell after the first
+  int32_t scan_cell[kScanCells];
+  int32_t scan_first[kScanCells];
+  unsigned long long scan_marks[kScanItems / 64];
 };
 
 // Pairs of the staged sites b [0, n) with the na real sites of i.
@@ -226,7 +234,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
   // occupancy prefix; cell 0 is i's own (partners j > i)
   const CellGrid& g = bv.cells;
   const bool cells = bv.mode == MCGPU_NB_CELL && g.overflow[0] == 0;
-  int ncell = 0, incl = 0, nitems = bv.n_mol - (i + 1), own_full = 0;
+  int ncell = 0, nfilled = 0, nitems = bv.n_mol - (i + 1), own_full = 0;
   int my_cell = 0;
   if (cells) {
     int co[3];
@@ -265,49 +273,66 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
       cnt = g.count[my_cell];
       cnt = cnt < g.cap ? cnt : g.cap;
     }
-    incl = wave_incl_scan_i32(cnt);
+    const int incl = wave_incl_scan_i32(cnt);
     nitems = readlane_i32(incl, 63);
+    const bool filled = lane < ncell && cnt > 0;
+    const unsigned long long fb = __ballot(filled);
+    nfilled = __popcll(fb);
+    if (filled) {
+      const int r = wave_rank(fb);
+      W.scan_cell[r] = my_cell | ((own_full || lane == 0) ? (int)0x80000000 : 0);
+      W.scan_first[r] = incl - cnt;
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-  // candidate of scan item `item` (-1 past the end) and whether it must
-  // exceed i: with cells, the cell whose inclusive prefix first exceeds the
-  // item (binary search over the lane-held prefix, all lanes taking part),
-  // then its slot; without, j = i + 1 + item
-  int span = 1;  // largest power of two <= ncell - 1, at least 1
-  while (2 * span <= ncell - 1) span <<= 1;
-  auto candidate = [&](int item, bool& greater) -> int {
-    if (!cells) {
-      greater = false;
-      return item < nitems ? i + 1 + item : -1;
-    }
-    int lo = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-      if (step > span) continue;  // wave-uniform
-      const int probe = lo + step - 1;
-      const int v = __shfl(incl, probe < 63 ? probe : 63);
-      if (probe < ncell && v <= item) lo += step;
-    }
-    const int before = __shfl(incl, lo > 0 ? lo - 1 : 0);
-    const int cell = __shfl(my_cell, lo < 63 ? lo : 63);
-    greater = own_full || lo == 0;
-    return item < nitems ? g.slots[cell * g.cap + (item - (lo > 0 ? before : 0))] : -1;
+  // Scan item -> candidate.  Without cells, j = i + 1 + item.  With cells,
+  // the items of the non-empty cells follow one another, so item t lies in
+  // cell r = the number of cells after the first whose first item is <= t:
+  // those first items are marked in a bitmap over a segment of kScanItems
+  // items, and each lane counts the marks up to its own item.
+  auto mark_segment = [&](int seg) -> int {
+    if (lane < kScanItems / 64) W.scan_marks[lane] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    const int f = lane >= 1 && lane < nfilled ? W.scan_first[lane] : 0x7fffffff;
+    if (f >= seg && f - seg < kScanItems)
+      atomicOr(&W.scan_marks[(f - seg) >> 6], 1ull << ((f - seg) & 63));
+    __builtin_amdgcn_wave_barrier();
+    return __popcll(__ballot(f < seg));  // cells starting before the segment
   };
   double e = 0.0;
   int nh = 0;
-  bool greater_next = false;
-  int j_next = nitems > 0 ? candidate(lane, greater_next) : -1;
-  for (int base = 0; base < nitems; base += 64) {
-    const int j = j_next;
-    const bool need_greater = greater_next;
-    if (base + 64 < nitems) j_next = candidate(base + 64 + lane, greater_next);
-    bool hit = false;
-    if (j >= 0 && j != i && (!need_greater || j > i)) {
+  int base = 0, seg = 0, seg_end = 0, before = 0;
+  do {
+    // windows of 64 items while the hit buffer has room for a whole window
+    for (; base < nitems && nh <= kWaveHits - 64; base += 64) {
+      const int item = base + lane;
+      int j = -1;
+      bool greater = false;
+      if (cells) {
+        if (base >= seg_end) {  // wave-uniform
+          seg = base;
+          seg_end = base + kScanItems;
+          before = mark_segment(seg);
+        }
+        const unsigned long long word = W.scan_marks[(base - seg) >> 6];
+        // marks at window positions 0..lane (2 << 63 wraps to 0: all bits)
+        const int r = before + __popcll(word & ((2ull << lane) - 1ull));
+        before += __popcll(word);
+        const int c = W.scan_cell[r];
+        greater = c < 0;
+        if (item < nitems) j = g.slots[(c & 0x7fffffff) * g.cap + (item - W.scan_first[r])];
+      } else if (item < nitems) {
+        j = i + 1 + item;
+      }
+      const bool ok = j >= 0 && j != i && (!greater || j > i);
+      bool hit = false;
       if (bv.single_primary) {
-        const double dx = wrap1(bv.px[j] - p0x, L0);
-        const double dy = wrap1(bv.py[j] - p0y, L1);
-        const double dz = wrap1(bv.pz[j] - p0z, L2);
-        hit = dx * dx + dy * dy + dz * dz <= bv.cutoff2;
-      } else {
+        const int jj = ok ? j : i;  // every lane loads a valid entry
+        const double dx = wrap1(bv.px[jj] - p0x, L0);
+        const double dy = wrap1(bv.py[jj] - p0y, L1);
+        const double dz = wrap1(bv.pz[jj] - p0z, L2);
+        hit = ok && dx * dx + dy * dy + dz * dz <= bv.cutoff2;
+      } else if (ok) {
         const int psj = bv.mol_pidx_start[j];
         const int pcj = bv.mol_pidx_count[j];
         for (int q = 0; q < pcj; q++) {
@@ -320,18 +345,16 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
           }
         }
       }
+      const unsigned long long bal = __ballot(hit);
+      if (hit) W.hit_start[nh + wave_rank(bal)] = j;
+      nh += __popcll(bal);
+      __builtin_amdgcn_wave_barrier();
     }
-    const unsigned long long bal = __ballot(hit);
-    const int cnt = __popcll(bal);
-    if (nh + cnt > kWaveHits) {  // wave-uniform
+    if (nh > 0) {  // wave-uniform
       e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np, p0x, p0y, p0z);
       nh = 0;
     }
-    if (hit) W.hit_start[nh + wave_rank(bal)] = j;
-    nh += cnt;
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (nh > 0) e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np, p0x, p0y, p0z);
+  } while (base < nitems);
   pairs = np;  // wave-uniform count
   return wave_sum_f64(e);
 }

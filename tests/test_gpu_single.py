@@ -101,6 +101,25 @@ def test_neighbor_structures_equal_brute_force(kind, skin):
     assert nb.system_energy() == pytest.approx(brute.system_energy(), rel=1e-12)
 
 
+@pytest.mark.parametrize("workload,kw", [
+    # 3 cells of 15.7 A per edge, ~130 molecules each: the half-shell scan
+    # of a molecule spans ~1800 candidates, more than one mapped segment
+    ("water256", dict(n_molecules=3468, edge=47.0, cutoff=12.0)),
+    # 2 cells per edge: every cell around the molecule, partners j > i
+    ("water256", {}),
+])
+def test_cell_full_energy_matches_oracle(workload, kw):
+    hb = m.workload(workload, **kw)
+    ob = O.OracleBox(hb.arrays())
+    db = m.DeviceBox(hb)
+    db.neighbor(m.NEIGHBOR_CELL)
+    e_cell = db.system_energy()
+    assert e_cell == pytest.approx(ob.system_energy(), rel=1e-12)
+    brute = m.DeviceBox(hb)
+    assert e_cell == pytest.approx(brute.system_energy(), rel=1e-12)
+    assert db.system_pairs() == brute.system_pairs()  # each pair once
+
+
 @pytest.mark.parametrize("kind,skin", [(m.NEIGHBOR_CELL, 0.0),
                                        (m.NEIGHBOR_VERLET, 1.0)])
 def test_neighbor_paths_keep_trace_parity(kind, skin):
