@@ -261,6 +261,8 @@ def main() -> int:
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--full-energy-only", action="store_true",
+                    help="measure only the full-energy kernel and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -271,6 +273,9 @@ def main() -> int:
     if m.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if args.full_energy_only:
+        print(json.dumps(full_energy_measure()), flush=True)
+        return 0
 
     t_setup = time.perf_counter()
     stream = make_stream(args.warmup + args.steps)
