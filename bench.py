@@ -21,7 +21,10 @@ batching server: value = trial moves / wall seconds.  Beside it:
     candidate primary, 24 B per site loaded) / HIP-event launch time;
   - the reference's own CPU path (oracle/_ref) timed on host cores on a
     bounded sample of the same jobs (kind "reference"), or the oracle
-    restatement when the reference build is absent (kind "port").
+    restatement when the reference build is absent (kind "port");
+  - the full-energy kernel's site-pair rate against its VALU roof, and the
+    per-move dE error of FP32 pair terms against FP64 on a box of each
+    configuration.
 """
 from __future__ import annotations
 
@@ -240,6 +243,51 @@ def full_energy_measure():
     return out
 
 
+def precision_error_measure(moves: int = 3000):
+    """Per-move |dE_mixed - dE_FP64| (kcal/mol) on a box of each
+    configuration: the same box and MC seed advanced with FP64 pair terms
+    and with FP32 pair terms in one batch, compared over the prefix of moves
+    on which both runs chose the same molecules and made the same
+    decisions (after the first differing decision the states differ)."""
+    import mcgpu_amd as m
+    cases = [("c0_water256", m.workload("water256"), m.NEIGHBOR_BRUTE),
+             ("c1_bicyclic4000", m.workload("bicyclic4000"), m.NEIGHBOR_CELL),
+             ("c3_mixture20000", m.workload("mixture20000"), m.NEIGHBOR_CELL)]
+    pool = BoxPool(seed=11)  # the serving stream's water generator
+    for n in (64, 256, 1024, 4096):
+        hb = pool.get(m.WATER, n)
+        cases.append((f"c2_water{n}", hb, neighbor_for(hb)))
+    out = {}
+    for name, hb, nb in cases:
+        b = m.Batch(2)
+        s64 = b.add(hb, steps=moves, seed=1, precision=m.FP64, neighbor=nb,
+                    record_trace=True)
+        s32 = b.add(hb, steps=moves, seed=1, precision=m.MIXED, neighbor=nb,
+                    record_trace=True)
+        for _ in range(1000):
+            if b.stats(s64).done and b.stats(s32).done:
+                break
+            b.run(2000)
+        t64, t32 = b.trace(s64), b.trace(s32)
+        b.close()
+        same = np.cumprod((t64.mol == t32.mol) & (t64.accepted == t32.accepted)).astype(bool)
+        k = int(same.sum())
+        err = np.abs(t32.de[:k] - t64.de[:k])
+        mag = np.abs(t64.de[:k])
+        # trial positions that overlap another molecule reach |dE| ~ 1e19
+        # (always rejected), where FP64 rounding alone is ~1e3 kcal/mol:
+        # the absolute maximum is reported over |dE| < 1e3, the relative
+        # one over all moves
+        phys = mag < 1e3
+        out[name] = dict(aligned_moves=k, moves=moves,
+                         median=float(np.median(err)) if k else None,
+                         p99=float(np.quantile(err, 0.99)) if k else None,
+                         max_abs_dE_below_1e3=float(err[phys].max()) if phys.any() else None,
+                         max_rel=float(np.max(err / np.maximum(mag, 1.0))) if k else None,
+                         frac_within_1e_4=float(np.mean(err <= 1e-4)) if k else None)
+    return out
+
+
 def pmc_traffic():
     """HBM bytes per launch of the batch kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
@@ -263,6 +311,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--full-energy-only", action="store_true",
                     help="measure only the full-energy kernel and print its JSON")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="measure only the FP32-pair dE error and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -275,6 +325,9 @@ def main() -> int:
         return 1
     if args.full_energy_only:
         print(json.dumps(full_energy_measure()), flush=True)
+        return 0
+    if args.precision_only:
+        print(json.dumps(precision_error_measure()), flush=True)
         return 0
 
     t_setup = time.perf_counter()
@@ -328,6 +381,8 @@ def main() -> int:
 
     full = full_energy_measure()
     print(f"# full energy: {json.dumps(full)}", file=sys.stderr, flush=True)
+    prec = precision_error_measure()
+    print(f"# FP32-pair dE error: {json.dumps(prec)}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -377,6 +432,7 @@ def main() -> int:
             "graph_launches_in_run": int(wst.ticks + st.ticks + cst.ticks),
         },
         "full_energy": full,
+        "fp32_pair_de_error_kcal_mol": prec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

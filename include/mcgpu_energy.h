@@ -44,8 +44,10 @@ extern "C" {
 
 /* ---- pair arithmetic ---- */
 #define MCGPU_FP64 0  /* every term and sum in FP64 */
-#define MCGPU_MIXED 1 /* pair LJ/Coulomb in FP32, displacements, sums,
-                         coordinates and Metropolis in FP64 */
+#define MCGPU_MIXED 1 /* 1/r estimate and LJ terms in FP32 (near-contact
+                         LJ in FP64); Coulomb from the FP32 1/r with one
+                         FP64 Newton step; displacements, sums, coordinates
+                         and Metropolis in FP64 */
 
 /* ---- neighbour structure used by energy calls ---- */
 #define MCGPU_NEIGHBOR_BRUTE 0  /* scan every molecule (BruteForceStep) */

@@ -186,25 +186,32 @@ __device__ __forceinline__ double pair_energy64(double dx, double dy, double dz,
   return r2 == 0.0 ? 0.0 : e;
 }
 
-// Mixed precision: displacement wrapped in FP64, pair terms in FP32, the
-// result handed back for FP64 accumulation.  Near-contact LJ pairs
-// (s^6 > 4, r < 0.79 sigma) are re-evaluated in FP64: there the FP32
-// rounding of s^12 (~20 ulp) would reach 1e-5..1e-3 kcal/mol per pair.
+// Mixed precision: displacement wrapped in FP64; 1/r estimated in FP32
+// (v_rsq_f32, ~1 ulp) and LJ computed from it in FP32; the result handed
+// back for FP64 accumulation.
+//  * Near-contact LJ pairs (s^6 > 4, r < 0.79 sigma) are re-evaluated in
+//    FP64: there the FP32 rounding of s^12 (~20 ulp) would reach
+//    1e-5..1e-3 kcal/mol per pair.
+//  * Coulomb takes the FP32 1/r refined by one FP64 Newton step (relative
+//    error ~1e-14) and an FP64 product.  Coulomb terms are ~10-100
+//    kcal/mol per pair, so FP32 products (~1 ulp each) summed over the
+//    ~2000 pairs of a water move put |dE| errors up to ~1.6e-4 kcal/mol;
+//    with this step the per-move error is that of the FP32 LJ terms alone.
 constexpr float kNearContactS6 = 4.0f;
 
 __device__ __forceinline__ double pair_energy32(double dx, double dy, double dz,
-                                                float sa, float ea, float qa,
-                                                float sb, float eb, float qb,
-                                                double sa64, double ea64,
-                                                double sb64, double eb64) {
+                                                float sa, float ea, float sb, float eb,
+                                                double sa64, double ea64, double qa64,
+                                                double sb64, double eb64, double qb64) {
   const double r2d = dx * dx + dy * dy + dz * dz;
   const float r2 = (float)r2d;
-  float rinv = __builtin_amdgcn_rsqf(r2);
-  rinv = rinv * (1.5f - 0.5f * r2 * rinv * rinv);  // one Newton step
+  const float rinv = __builtin_amdgcn_rsqf(r2);
   const float s2 = (sa * sb) * (rinv * rinv);
   const float s6 = s2 * s2 * s2;
   const float eps = ea * eb;
-  const float coul = (qa * qb) * rinv;
+  double rd = (double)rinv;
+  rd = rd * fma(-0.5 * r2d * rd, rd, 1.5);
+  const double coul = (qa64 * qb64) * rd;
   double lj;
   if (s6 > kNearContactS6 && eps != 0.0f) {
     const double s2d = (sa64 * sb64) / r2d;
@@ -213,7 +220,7 @@ __device__ __forceinline__ double pair_energy32(double dx, double dy, double dz,
   } else {
     lj = (double)(eps * (s6 * s6 - s6));
   }
-  return r2d == 0.0 ? 0.0 : lj + (double)coul;
+  return r2d == 0.0 ? 0.0 : lj + coul;
 }
 
 // ------------------------------------------------------- wave / block ops

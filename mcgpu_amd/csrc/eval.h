@@ -168,7 +168,7 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
   const double xo = sh.ox[ac], yo = sh.oy[ac], zo = sh.oz[ac];
   const double xn = sh.nx[ac], yn = sh.ny[ac], zn = sh.nz[ac];
   const double sa = sh.q64[ac].s, ea = sh.q64[ac].e, qa = sh.q64[ac].q;
-  const float sa32 = sh.q32[ac].s, ea32 = sh.q32[ac].e, qa32 = sh.q32[ac].q;
+  const float sa32 = sh.q32[ac].s, ea32 = sh.q32[ac].e;
   const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
   const int chunk = CAP / M;
   for (int h0 = 0; h0 < nhits; h0 += chunk) {
@@ -234,13 +234,13 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
           const SiteParam32 pb = sh.prm32[pk];
           const SiteParam64& pd = sh.prm64[pk];
           if (f[v] & 1) {
-            e_old += pair_energy32(dxo, dyo, dzo, sa32, ea32, qa32, pb.s, pb.e, pb.q,
-                                   sa, ea, pd.s, pd.e);
+            e_old += pair_energy32(dxo, dyo, dzo, sa32, ea32, pb.s, pb.e, sa, ea, qa, pd.s,
+                                   pd.e, pd.q);
             npairs++;
           }
           if (f[v] & 2)
-            e_new += pair_energy32(dxn, dyn, dzn, sa32, ea32, qa32, pb.s, pb.e, pb.q,
-                                   sa, ea, pd.s, pd.e);
+            e_new += pair_energy32(dxn, dyn, dzn, sa32, ea32, pb.s, pb.e, sa, ea, qa, pd.s,
+                                   pd.e, pd.q);
         } else {
           const SiteParam64 pb = sh.prm64[pk];
           if (f[v] & 1) {

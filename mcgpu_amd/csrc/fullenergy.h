@@ -14,8 +14,12 @@ namespace mcgpu {
 // molecule i and no barriers between waves:
 //  * the lanes scan i's candidate partners -- with cell lists, i's own cell
 //    (partners j > i) and the 13 neighbour cells on one side of it (any j),
-//    so that every pair of cells is visited from one end only; without, all
-//    j > i -- and compact the hits into the wave's LDS list by ballot;
+//    so that every pair of cells is visited from one end only, or, with
+//    fewer than 3 cells along an axis, every cell around i (j > i); without
+//    cell lists, all j > i -- and compact the hits' ids into the wave's LDS
+//    list by ballot;
+//  * one lane per hit reads the partner's record and classifies the pair's
+//    periodic image;
 //  * the hits' real sites are staged densely in LDS (coordinates already
 //    carrying the pair's periodic image), and each lane takes one staged
 //    site b against every real site a of i, whose coordinates and
@@ -74,8 +78,8 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
         const SiteParam64& A = W.ap[a];
         if (MIXED) {
           const SiteParam32& A32 = W.ap32[a];
-          e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e,
-                             B.s, B.e);
+          e += pair_energy32(dx, dy, dz, A32.s, A32.e, B32.s, B32.e, A.s, A.e, A.q, B.s, B.e,
+                             B.q);
         } else {
           e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
         }
@@ -87,8 +91,8 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
         const SiteParam64& A = W.ap[a];
         if (MIXED) {
           const SiteParam32& A32 = W.ap32[a];
-          e += pair_energy32(dx, dy, dz, A32.s, A32.e, A32.q, B32.s, B32.e, B32.q, A.s, A.e,
-                             B.s, B.e);
+          e += pair_energy32(dx, dy, dz, A32.s, A32.e, B32.s, B32.e, A.s, A.e, A.q, B.s, B.e,
+                             B.q);
         } else {
           e += pair_energy64(dx, dy, dz, A.s, A.e, A.q, B.s, B.e, B.q);
         }

@@ -116,7 +116,7 @@ def test_batch_mixed_precision_error():
     print(f"batch mixed water256: {n}/{steps} aligned, max {err.max():.3e}, "
           f"p99 {np.quantile(err, 0.99):.3e}, median {np.median(err):.3e}")
     assert n >= 2000
-    assert np.quantile(err, 0.99) <= 1e-4
+    assert err.max() <= 1e-4
 
 
 def test_batch_npt_energy_stays_consistent():

@@ -136,8 +136,9 @@ def test_neighbor_paths_keep_trace_parity(kind, skin):
 
 
 def test_mixed_precision_error_bound():
-    """FP32 pair terms, FP64 sums: per-move |dE - dE_fp64| over a trajectory
-    where both make the same decisions."""
+    """FP32 LJ terms and FP32-estimated 1/r, FP64 sums: per-move
+    |dE - dE_fp64| over a trajectory where both make the same decisions,
+    within 1e-4 kcal/mol on every move."""
     ref = np.load(GOLDEN / "water256.ref.npz")
     steps = 10000
     res, tr = m.plain_run(load("water256"), steps, precision=m.MIXED)
@@ -147,8 +148,7 @@ def test_mixed_precision_error_bound():
     print(f"mixed water256: {n} moves aligned, max |dE err| {err.max():.3e}, "
           f"p99 {np.quantile(err, 0.99):.3e}")
     assert n >= 1000
-    assert err.max() <= 1e-3
-    assert np.quantile(err, 0.99) <= 1e-4
+    assert err.max() <= 1e-4
 
 
 @pytest.mark.parametrize("f", [1.03, 0.98])
