@@ -28,10 +28,11 @@ namespace mcgpu {
 // elsewhere, so the total is deterministic.  Which molecule of a pair holds
 // its terms differs from the reference's E_i(i) split (j > i); the set of
 // pairs, and so the total up to summation order, is the same.
-constexpr int kWaveHits = 128;   // hits buffered before their pairs are summed
+constexpr int kWaveHits = 192;   // hits buffered before their pairs are summed
 constexpr int kWaveItems = 128;  // staged sites b per pass of the pair loop
 constexpr int kScanItems = 1024; // scan items mapped per segment (cell lists)
 constexpr int kScanCells = 32;   // cells of one scan (14 half-shell, <= 27 full)
+static_assert(kParamCap <= 256 && kMaxSites < 256, "hit_info packs both in 8 bits");
 
 struct WaveEval {
   // real sites of molecule i (compacted): coordinates, FP64 and FP32 pair
@@ -40,9 +41,8 @@ struct WaveEval {
   SiteParam64 ap[kMaxSites];
   SiteParam32 ap32[kMaxSites];
   int32_t hit_start[kWaveHits];  // partner id until wave_resolve_hits
-  int32_t hit_param[kWaveHits];
-  int32_t hit_len[kWaveHits];
-  int32_t hit_img[kWaveHits];  // periodic image of the pair (image_code)
+  // parameter entry | site count << 8 | periodic image (image_code) << 16
+  int32_t hit_info[kWaveHits];
   // staged real sites b of the hits: coordinates and parameter entry, with
   // kPairWrap set when the hit's site pairs may wrap differently
   double bx[kWaveItems], by[kWaveItems], bz[kWaveItems];
@@ -112,17 +112,15 @@ __device__ __forceinline__ void wave_resolve_hits(const BoxView& bv, WaveEval& W
   for (int h = wlane(); h < nh; h += 64) {
     const int j = W.hit_start[h];
     const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
-    W.hit_img[h] = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
-                              bv.reach2);
+    const int img = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
+                               bv.reach2);
     if (bv.uni_len > 0) {
       W.hit_start[h] = j * bv.uni_len;
-      W.hit_len[h] = bv.uni_len;
-      W.hit_param[h] = bv.uni_param;
+      W.hit_info[h] = bv.uni_param | bv.uni_len << 8 | img << 16;
     } else {
       const MolMeta m = bv.meta[j];
       W.hit_start[h] = m.start;
-      W.hit_len[h] = m.len;
-      W.hit_param[h] = m.param;
+      W.hit_info[h] = m.param | m.len << 8 | img << 16;
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -158,11 +156,12 @@ __device__ __forceinline__ double wave_hits_energy(const BoxView& bv, WaveEval& 
     const int k = k0 + kk;
     bool ok = false;
     int pb = 0, site = 0, img = 0;
-    if (kk < per_pass && k < nh && b < W.hit_len[k]) {
-      pb = W.hit_param[k] + b;
+    const int info = kk < per_pass && k < nh ? W.hit_info[k] : 0;
+    if (kk < per_pass && k < nh && b < ((info >> 8) & 0xff)) {
+      pb = (info & 0xff) + b;
       ok = P64[pb].real != 0.0;
       site = W.hit_start[k] + b;
-      img = W.hit_img[k];
+      img = info >> 16;
     }
     const unsigned long long bal = __ballot(ok);
     if (ok) {
