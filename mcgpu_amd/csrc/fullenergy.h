@@ -43,6 +43,7 @@ struct WaveEval {
   int32_t hit_start[kWaveHits];  // partner id until wave_resolve_hits
   // parameter entry | site count << 8 | periodic image (image_code) << 16
   int32_t hit_info[kWaveHits];
+  uint16_t hit_first[kWaveHits];  // first item of the hit's sites in the run
   // staged real sites b of the hits: coordinates and parameter entry, with
   // kPairWrap set when the hit's site pairs may wrap differently
   double bx[kWaveItems], by[kWaveItems], bz[kWaveItems];
@@ -106,58 +107,106 @@ __device__ __forceinline__ double wave_items_energy(const BoxView& bv, const Wav
 // each into its record (first site, site count, parameter entry) and the
 // pair's periodic image, classified from the two first primaries (p0 is
 // i's).  Doing this once per hit rather than once per scanned candidate
-// keeps the scan loop to loads and the cutoff test.
-__device__ __forceinline__ void wave_resolve_hits(const BoxView& bv, WaveEval& W, int nh,
-                                                  double p0x, double p0y, double p0z) {
-  for (int h = wlane(); h < nh; h += 64) {
-    const int j = W.hit_start[h];
-    const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
-    const int img = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z, bv.L,
-                               bv.reach2);
-    if (bv.uni_len > 0) {
-      W.hit_start[h] = j * bv.uni_len;
-      W.hit_info[h] = bv.uni_param | bv.uni_len << 8 | img << 16;
-    } else {
-      const MolMeta m = bv.meta[j];
-      W.hit_start[h] = m.start;
-      W.hit_info[h] = m.param | m.len << 8 | img << 16;
+// keeps the scan loop to loads and the cutoff test.  The hits' sites are
+// numbered in one run (hit_first: prefix of the site counts); returns the
+// run's length.
+__device__ __forceinline__ int wave_resolve_hits(const BoxView& bv, WaveEval& W, int nh,
+                                                 double p0x, double p0y, double p0z) {
+  const int lane = wlane();
+  int first = 0;
+  for (int h0 = 0; h0 < nh; h0 += 64) {  // wave-uniform
+    const int h = h0 + lane;
+    int len = 0;
+    if (h < nh) {
+      const int j = W.hit_start[h];
+      const int psj = bv.single_primary ? j : bv.mol_pidx_start[j];
+      const int img = image_code(bv.px[psj] - p0x, bv.py[psj] - p0y, bv.pz[psj] - p0z,
+                                 bv.L, bv.reach2);
+      int param;
+      if (bv.uni_len > 0) {
+        W.hit_start[h] = j * bv.uni_len;
+        len = bv.uni_len;
+        param = bv.uni_param;
+      } else {
+        const MolMeta m = bv.meta[j];
+        W.hit_start[h] = m.start;
+        len = m.len;
+        param = m.param;
+      }
+      W.hit_info[h] = param | len << 8 | img << 16;
     }
+    const int incl = wave_incl_scan_i32(len);
+    if (h < nh) W.hit_first[h] = (uint16_t)(first + incl - len);
+    first += readlane_i32(incl, 63);
   }
   __builtin_amdgcn_wave_barrier();
+  return first;
 }
 
-// Sums the pairs of hits [0, nh) with i: their real sites are staged
-// kWaveItems at a time (whole hits per pass of 64 slots, compacted by
-// ballot), each pass followed by the pair loop.  npairs counts the site
-// pairs.
+// Sums the pairs of hits [0, nh) with i.  The run of the hits' sites is
+// staged 64 sites per pass: lane t takes site t of the run, the hit holding
+// it found by counting, with one popcount, the marks of a bitmap of the
+// hits' first sites (segments of kScanItems sites; it reuses the scan's
+// bitmap, which the scan rebuilds afterwards).  Real sites are compacted by
+// ballot; whenever the buffer could overflow, the pair loop takes the whole
+// 64-site passes and the remainder moves to the front.  npairs counts the
+// site pairs.
 template <bool MIXED>
 __device__ __forceinline__ double wave_hits_energy(const BoxView& bv, WaveEval& W, int na,
                                                    int nh, const SiteParam64* P64,
                                                    const SiteParam32* P32, unsigned& npairs,
                                                    double p0x, double p0y, double p0z) {
-  wave_resolve_hits(bv, W, nh, p0x, p0y, p0z);
-  const int M = bv.max_len;
-  const int per_pass = 64 / M;  // whole hits per staging pass
-  const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
+  const int nsite = wave_resolve_hits(bv, W, nh, p0x, p0y, p0z);
   const double L0 = bv.L[0], L1 = bv.L[1], L2 = bv.L[2];
   const int lane = wlane();
   double e = 0.0;
-  int n = 0;
-  for (int k0 = 0; k0 < nh; k0 += per_pass) {
+  int n = 0, seg = 0, seg_end = 0, before = 0;
+  for (int base = 0; base < nsite; base += 64) {
     if (n + 64 > kWaveItems) {  // wave-uniform
+      const int full = n & ~63;
       __builtin_amdgcn_wave_barrier();
-      e += wave_items_energy<MIXED>(bv, W, na, n, P64, P32);
-      npairs += (unsigned)(n * na);
+      e += wave_items_energy<MIXED>(bv, W, na, full, P64, P32);
+      npairs += (unsigned)(full * na);
       __builtin_amdgcn_wave_barrier();
-      n = 0;
+      const int rest = n - full;
+      if (lane < rest) {
+        W.bx[lane] = W.bx[full + lane];
+        W.by[lane] = W.by[full + lane];
+        W.bz[lane] = W.bz[full + lane];
+        W.bp[lane] = W.bp[full + lane];
+      }
+      __builtin_amdgcn_wave_barrier();
+      n = rest;
     }
-    const int kk = div_small(lane, magic);
-    const int b = lane - kk * M;
-    const int k = k0 + kk;
+    if (base >= seg_end) {  // wave-uniform
+      seg = base;
+      seg_end = base + kScanItems;
+      if (lane < kScanItems / 64) W.scan_marks[lane] = 0ull;
+      __builtin_amdgcn_wave_barrier();
+      before = 0;
+      for (int h0 = 0; h0 < nh; h0 += 64) {
+        const int h = h0 + lane;
+        bool pre = false;
+        if (h >= 1 && h < nh) {
+          const int f = W.hit_first[h];
+          pre = f < seg;
+          if (!pre && f - seg < kScanItems)
+            atomicOr(&W.scan_marks[(f - seg) >> 6], 1ull << ((f - seg) & 63));
+        }
+        before += __popcll(__ballot(pre));
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    const unsigned long long word = W.scan_marks[(base - seg) >> 6];
+    // marks at run positions base..base+lane (2 << 63 wraps to 0: all bits)
+    const int k = before + __popcll(word & ((2ull << lane) - 1ull));
+    before += __popcll(word);
+    const int item = base + lane;
     bool ok = false;
     int pb = 0, site = 0, img = 0;
-    const int info = kk < per_pass && k < nh ? W.hit_info[k] : 0;
-    if (kk < per_pass && k < nh && b < ((info >> 8) & 0xff)) {
+    if (item < nsite) {
+      const int info = W.hit_info[k];
+      const int b = item - W.hit_first[k];
       pb = (info & 0xff) + b;
       ok = P64[pb].real != 0.0;
       site = W.hit_start[k] + b;
@@ -356,6 +405,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     if (nh > 0) {  // wave-uniform
       e += wave_hits_energy<MIXED>(bv, W, na, nh, P64, P32, np, p0x, p0y, p0z);
       nh = 0;
+      seg_end = 0;  // the staging reused the scan's bitmap: map again
     }
   } while (base < nitems);
   pairs = np;  // wave-uniform count
