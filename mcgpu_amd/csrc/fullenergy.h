@@ -32,6 +32,12 @@ constexpr int kWaveHits = 192;   // hits buffered before their pairs are summed
 constexpr int kWaveItems = 128;  // staged sites b per pass of the pair loop
 constexpr int kScanItems = 1024; // scan items mapped per segment (cell lists)
 constexpr int kScanCells = 32;   // cells of one scan (14 half-shell, <= 27 full)
+// Boxes whose largest molecule has at least this many sites split the pairs
+// by index parity over all 27 cells instead of the half shell: a molecule's
+// share of the pair work then varies far less (per-SIMD maximum over mean
+// 1.17 against 1.63 on a 4,000-molecule 16-site box), which pays for
+// scanning twice the cells when each hit carries ~100+ site pairs.
+constexpr int kParityLen = 8;
 static_assert(kParamCap <= 256 && kMaxSites < 256, "hit_info packs both in 8 bits");
 
 struct WaveEval {
@@ -288,12 +294,28 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
   const bool cells = bv.mode == MCGPU_NB_CELL && g.overflow[0] == 0;
   int ncell = 0, nfilled = 0, nitems = bv.n_mol - (i + 1), own_full = 0;
   int my_cell = 0;
+  bool parity = false;
   if (cells) {
     int co[3];
     co[0] = cell_coord(p0x, g.inv_w[0], g.n[0]);
     co[1] = cell_coord(p0y, g.inv_w[1], g.n[1]);
     co[2] = cell_coord(p0z, g.inv_w[2], g.n[2]);
-    if (g.n[0] >= 3 && g.n[1] >= 3 && g.n[2] >= 3) {
+    parity = g.n[0] >= 3 && g.n[1] >= 3 && g.n[2] >= 3 && bv.max_len >= kParityLen;
+    if (parity) {
+      // all 27 cells around i; i holds the pair with j when i + j is even
+      // and j > i, or odd and j < i (every pair once, from either end)
+      ncell = 27;
+      if (lane < 27) {
+        int a = co[0] + lane / 9 - 1, b = co[1] + (lane / 3) % 3 - 1, c = co[2] + lane % 3 - 1;
+        a += a < 0 ? g.n[0] : 0;
+        a -= a >= g.n[0] ? g.n[0] : 0;
+        b += b < 0 ? g.n[1] : 0;
+        b -= b >= g.n[1] ? g.n[1] : 0;
+        c += c < 0 ? g.n[2] : 0;
+        c -= c >= g.n[2] ? g.n[2] : 0;
+        my_cell = cell_index(g, a, b, c);
+      }
+    } else if (g.n[0] >= 3 && g.n[1] >= 3 && g.n[2] >= 3) {
       // own cell + the 13 offsets (dx, dy, dz) > (0, 0, 0) lexicographically
       ncell = 14;
       if (lane < 14) {
@@ -332,7 +354,7 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
     nfilled = __popcll(fb);
     if (filled) {
       const int r = wave_rank(fb);
-      W.scan_cell[r] = my_cell | ((own_full || lane == 0) ? (int)0x80000000 : 0);
+      W.scan_cell[r] = my_cell | ((!parity && (own_full || lane == 0)) ? (int)0x80000000 : 0);
       W.scan_first[r] = incl - cnt;
     }
     __builtin_amdgcn_wave_barrier();
@@ -376,7 +398,8 @@ __device__ __forceinline__ double wave_molecule_energy(const BoxView& bv, WaveEv
       } else if (item < nitems) {
         j = i + 1 + item;
       }
-      const bool ok = j >= 0 && j != i && (!greater || j > i);
+      const bool ok = j >= 0 && j != i &&
+                      (parity ? (((i ^ j) & 1) == 0) == (j > i) : (!greater || j > i));
       bool hit = false;
       if (bv.single_primary) {
         const int jj = ok ? j : i;  // every lane loads a valid entry

@@ -107,6 +107,9 @@ def test_neighbor_structures_equal_brute_force(kind, skin):
     ("water256", dict(n_molecules=3468, edge=47.0, cutoff=12.0)),
     # 2 cells per edge: every cell around the molecule, partners j > i
     ("water256", {}),
+    # 16-site molecules, 5 cells per edge: all 27 cells, pairs split by
+    # index parity
+    ("bicyclic4000", dict(n_molecules=1500)),
 ])
 def test_cell_full_energy_matches_oracle(workload, kw):
     hb = m.workload(workload, **kw)
