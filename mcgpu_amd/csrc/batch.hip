@@ -8,8 +8,9 @@
 // independent, so a launch is as long as its slowest box.
 //
 // Box memory is carved from one HBM arena (admission by footprint).  NPT
-// volume moves run between launches with the multi-work-group full-energy
-// kernel (see batch_volume_move).
+// volume moves run between launches, decided on the device by a short
+// sequence of multi-box kernels around the full-energy pass (see
+// enqueue_volume_moves).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -148,7 +148,9 @@ __global__ __launch_bounds__(NT) void k_verlet_build(BoxView bv, double rv2) {
 }
 
 // Full-system energy of the box (SimulationStep.cpp:40-71), one wave per
-// molecule (see fullenergy.h): partial[i] = E_i(i), pairs[i] = site pairs.
+// molecule (see fullenergy.h): partial[i] = the energy of the pairs molecule
+// i holds (every pair is held by one of its molecules), pairs[i] = their
+// site pairs.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_system_waves(BoxView bv, double* partial,
                                                       double* pairs) {

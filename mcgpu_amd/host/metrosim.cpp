@@ -69,7 +69,7 @@ void help() {
       "--parallel\t(-p)\tRun on the GPU (the only engine of this build).\n"
       "--serial\t(-s)\tAccepted for compatibility: also runs on the GPU.\n"
       "--verbose\t(-k)\tPrint status updates.\n"
-      "--mixed\t\t\tFP32 pair terms, FP64 sums and Metropolis test.\n\n"
+      "--mixed\t\t\tFP32 LJ terms and 1/r estimates; FP64 sums and Metropolis test.\n\n"
       "Simulation Run Parameters\n==========================\n"
       "--name <title>\t\tBase name of the .results, .pdb and .state files.\n"
       "--steps <count>\t(-n)\tTrial moves to run (> 0).\n"
