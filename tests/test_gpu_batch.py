@@ -136,6 +136,26 @@ def test_batch_npt_energy_stays_consistent():
     assert st.energy == pytest.approx(e, rel=1e-9, abs=1e-6)
 
 
+def test_batch_npt_large_molecules_with_cells():
+    """NPT on 16-site molecules with 5 cells per edge: the volume moves'
+    full energies take the parity pair split over all 27 cells; the
+    tracked energy must equal a fresh oracle evaluation of the final box."""
+    hb = m.workload("bicyclic4000", n_molecules=1500)
+    steps = 1000
+    b = m.Batch(2)
+    s = b.add(hb, steps=steps, ensemble=m.NPT, volume_interval=250,
+              pressure=1.4584e-5, max_volume_delta=0.02, neighbor=m.NEIGHBOR_CELL)
+    (st,) = run_all(b, [s], steps, 250)
+    assert st.steps_done == steps and st.done
+    assert st.volume_moves == 3  # at 250, 500 and 750 moves
+    x, y, z = b.download(s)
+    a = hb.arrays()
+    a.update(x=x, y=y, z=z, box=np.array(st.box[:]))
+    ob = O.OracleBox(a)
+    e = ob.system_energy() + ob.lrc_reference()
+    assert st.energy == pytest.approx(e, rel=1e-9, abs=1e-6)
+
+
 def test_batch_admission_by_footprint():
     hb = load("water256")
     import ctypes as C
