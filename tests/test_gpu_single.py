@@ -110,6 +110,9 @@ def test_neighbor_structures_equal_brute_force(kind, skin):
     # 16-site molecules, 5 cells per edge: all 27 cells, pairs split by
     # index parity
     ("bicyclic4000", dict(n_molecules=1500)),
+    # the full-size water/methanol NPT workload (20,000 molecules, 3 and 6
+    # sites, 8 cells per edge)
+    ("mixture20000", {}),
 ])
 def test_cell_full_energy_matches_oracle(workload, kw):
     hb = m.workload(workload, **kw)
