@@ -839,6 +839,7 @@ struct mcgpu_batch {
   Rng31* h_rng_rb = nullptr;    // pinned read-back of a tick
   int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
   int32_t* h_active = nullptr;  // pinned     [count, slots...]
+  std::vector<int32_t> next_active;  // the lists of the tick being set up
   VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
   VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
@@ -1172,6 +1173,13 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
   }
   std::memset(b->h_table, 0, sizeof(BatchBox) * M);
   std::memset(b->h_ctl, 0, sizeof(BoxCtl) * M);
+  // empty active lists on both sides: a tick uploads a group's list only
+  // when it differs from what the device holds (h_active mirrors d_active)
+  std::memset(b->h_active, 0, sizeof(int32_t) * kGroups * (M + 1));
+  if (hipMemset(b->d_active, 0, sizeof(int32_t) * kGroups * (M + 1)) != hipSuccess) {
+    set_error("mcgpu_batch_create: allocation failed");
+    return fail(MCGPU_ERR_HIP);
+  }
   std::memset(b->h_rng, 0, sizeof(Rng31) * M);
   for (size_t s = 0; s < M; s++) b->h_ctl[s].status = kDone;
   {
@@ -1461,10 +1469,12 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   int n[kGroups] = {};
   int total = 0;
   b->launched.clear();
+  std::vector<int32_t>& act = b->next_active;
+  act.assign((size_t)kGroups * stride, 0);
   for (int s = 0; s < b->max_boxes; s++) {
     const BatchSlot& S = b->slots[s];
     if (S.used && b->h_ctl[s].status == kRunning) {
-      b->h_active[S.group * stride + 1 + n[S.group]++] = s;
+      act[S.group * stride + 1 + n[S.group]++] = s;
       b->launched.push_back(s);
       b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
     }
@@ -1475,11 +1485,16 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   // set-up work queued so far (admissions, decided volume moves) first
   MCGPU_HIP_TRY(hipEventRecord(b->ev_ready, b->sstream));
   MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_ready, 0));
+  // a group's list goes up only when it changed (the previous tick, and so
+  // its upload from h_active, has completed: batch_boundary waited for it)
   for (int g = 0; g < kGroups; g++) {
-    b->h_active[g * stride] = n[g];
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, b->h_active + g * stride,
-                                 sizeof(int32_t) * (1 + n[g]), hipMemcpyHostToDevice,
-                                 b->stream));
+    act[g * stride] = n[g];
+    const size_t words = 1 + (size_t)n[g];
+    int32_t* h = b->h_active + g * stride;
+    if (std::memcmp(h, &act[g * stride], sizeof(int32_t) * words) == 0) continue;
+    std::memcpy(h, &act[g * stride], sizeof(int32_t) * words);
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, h, sizeof(int32_t) * words,
+                                 hipMemcpyHostToDevice, b->stream));
   }
   // grids rounded up (surplus work-groups exit at once), so the captured
   // graph survives most admissions and retirements
@@ -1523,10 +1538,13 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
   MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
-  // read-back of the boxes' control blocks and generators into staging
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl_rb, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
+  // read-back of the launched boxes' control blocks and generators into
+  // staging (slots below the highest launched one; the boundary merges the
+  // launched slots only)
+  const size_t nrb = (size_t)b->launched.back() + 1;
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl_rb, b->d_ctl, sizeof(BoxCtl) * nrb,
                                hipMemcpyDeviceToHost, b->stream));
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng_rb, b->d_rng, sizeof(Rng31) * b->max_boxes,
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng_rb, b->d_rng, sizeof(Rng31) * nrb,
                                hipMemcpyDeviceToHost, b->stream));
   b->pending = true;
   b->timing_pending = true;
