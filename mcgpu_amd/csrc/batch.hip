@@ -1414,10 +1414,10 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   }
   // start energy: the full-energy pass over this one box
   if (p->precision == MCGPU_MIXED)
-    hipLaunchKernelGGL(k_energy_waves<true>, dim3(std::min((v.n_mol + 3) / 4, 2048), 1),
+    hipLaunchKernelGGL(k_energy_waves<true>, dim3((v.n_mol + 3) / 4, 1),
                        dim3(256), 0, st, b->d_table, b->d_one + s);
   else
-    hipLaunchKernelGGL(k_energy_waves<false>, dim3(std::min((v.n_mol + 3) / 4, 2048), 1),
+    hipLaunchKernelGGL(k_energy_waves<false>, dim3((v.n_mol + 3) / 4, 1),
                        dim3(256), 0, st, b->d_table, b->d_one + s);
   hipLaunchKernelGGL(k_reduce_multi, dim3(1), dim3(256), 0, st, b->d_table, b->d_one + s, 0,
                      E.total + 2);

@@ -853,7 +853,9 @@ extern "C" int mcgpu_trial_move(mcgpu_box* b, int32_t mol,
 extern "C" int mcgpu_system_energy(mcgpu_box* b, double* energy) {
   if (!b || !energy) return MCGPU_ERR_INVALID;
   const int n = b->bv.n_mol;
-  const int grid = std::min((n + 3) / 4, 2048);
+  // one molecule per wave: the hardware hands out blocks as others finish,
+  // which evens out molecules of unequal pair work
+  const int grid = (n + 3) / 4;
   if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   if (b->precision == MCGPU_MIXED)
     hipLaunchKernelGGL(k_system_waves<true>, dim3(grid), dim3(256), 0, b->stream, b->bv,
