@@ -927,7 +927,7 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
 // energies into d_volout[2 i + which]; cell lists for the current geometry.
 int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
                        int which, hipStream_t st, bool any_fp64, bool any_mixed) {
-  const dim3 grid(std::min((max_mol + 3) / 4, 2048), k);
+  const dim3 grid((max_mol + 3) / 4, k);  // blocks past a box's molecules exit at once
   if (any_fp64)
     hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
   if (any_mixed)

@@ -442,6 +442,7 @@ template <bool MIXED>
 __device__ __forceinline__ void full_energy_block(const BoxView& bv, double* partial,
                                                   double* pairs, WaveEval* ws,
                                                   SiteParam64* p64, SiteParam32* p32) {
+  if ((int)blockIdx.x * 4 >= bv.n_mol) return;  // the whole block: no molecule
   for (int k = threadIdx.x; k < bv.n_param; k += 256) {
     p64[k] = bv.p64[k];
     p32[k] = bv.p32[k];
