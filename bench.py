@@ -316,8 +316,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (boxes would shard across GPUs with no exchange; see the "
-              "design notes)", file=sys.stderr)
+              "supported (independent boxes would shard across GPUs with no "
+              "exchange between them)", file=sys.stderr)
         return 2
     import mcgpu_amd as m
     if m.device_count() < 1:
