@@ -9,10 +9,12 @@ import numpy as np
 import pytest
 
 import mcgpu_amd as m
+from edge_boxes import build_box, kind
 from oracle import oracle as O
 
 ROOT = Path(__file__).resolve().parents[1]
 GOLDEN = ROOT / "tests" / "golden"
+ERR_INVALID, ERR_NODEVICE = -1, -6   # MCGPU_ERR_INVALID, MCGPU_ERR_NODEVICE
 
 
 def declared_symbols():
@@ -52,6 +54,53 @@ def test_invalid_descriptor_rejected_before_device():
     assert m.lib().mcgpu_box_create(C.byref(d), 0, C.byref(b)) == -1
     d = hb.desc()
     assert m.lib().mcgpu_box_create(C.byref(d), 7, C.byref(b)) == -1
+
+
+def _edge_case(case):
+    """A descriptor at or past one limit, built from plain arrays."""
+    rng = np.random.default_rng(5)
+    if case == "too_many_sites":        # 33 > MCGPU_MAX_MOL_SITES
+        return build_box([kind(33, rng, (0,))], 8, 30.0, 10.0, 1), "molecule length"
+    if case == "too_many_params":       # 5 x 32 distinct entries > 128
+        kinds = [kind(32, rng, (0,)) for _ in range(5)]
+        return build_box(kinds, 10, 40.0, 10.0, 1), "site-parameter"
+    ab = build_box([kind(4, rng, (1,))], 27, 30.0, 10.0, 1)
+    if case == "no_molecules":
+        ab.a["mol_start"] = ab.a["mol_start"][:0]
+        return ab, "n_molecules"
+    if case == "primary_outside":
+        ab.a["primary_index"][3] = 0    # molecule 3's primary in molecule 0
+        return ab, "primary index outside"
+    if case == "nan_coordinate":
+        ab.a["y"][17] = np.nan
+        return ab, "non-finite"
+    if case == "ragged_ok":             # 1- to 32-site molecules: accepted
+        return build_box([kind(1, rng, (0,)), kind(32, rng, (0, 31))], 16, 30.0, 10.0, 1), None
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["too_many_sites", "too_many_params", "no_molecules",
+                                  "primary_outside", "nan_coordinate", "ragged_ok"])
+def test_descriptor_limits(case):
+    """Descriptors past a limit are refused before any device work, with a
+    message naming the limit; one at the limits is accepted."""
+    ab, why = _edge_case(case)
+    d = ab.desc()
+    b = C.c_void_p()
+    rc = m.lib().mcgpu_box_create(C.byref(d), 0, C.byref(b))
+    if why is None:
+        nbytes = C.c_size_t()
+        p = m.RunParams(1, 0, 0, 0, 100, 0.0, 0.0, 0.0, 0, 0)
+        assert m.lib().mcgpu_box_footprint(C.byref(d), C.byref(p), C.byref(nbytes)) == 0
+        assert nbytes.value > 0
+        if m.device_count() == 0:
+            assert rc == ERR_NODEVICE
+        else:
+            assert rc == 0
+            m.lib().mcgpu_box_destroy(b)
+        return
+    assert rc == ERR_INVALID
+    assert why in m.lib().mcgpu_last_error().decode()
 
 
 def test_generators_are_deterministic_and_sized():

@@ -4,116 +4,15 @@ primary sites per molecule at a non-zero offset, dummy sites, and ragged
 boxes that mix 1-, 2-, 5- and 32-site molecules.  Boxes are built here from
 plain arrays (the loader's descriptor layout), checked against the oracle's
 energies and against its trial-move traces for the same seed."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
 import mcgpu_amd as m
+from edge_boxes import max_site_box, ragged_box
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 DE_TOL = 1e-9
-
-
-class ArrayBox:
-    """A box described by numpy arrays; duck-types the HostBox methods that
-    DeviceBox and Batch.add use (desc, mc_seed)."""
-
-    def __init__(self, a: dict, mc_seed: int):
-        self.a = a
-        self.mc_seed = mc_seed
-
-    def arrays(self):
-        return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.a.items()}
-
-    def desc(self) -> m.BoxDesc:
-        a = self.a
-        d = m.BoxDesc()
-        d.n_molecules = len(a["mol_start"])
-        d.n_sites = len(a["x"])
-        d.n_primary = len(a["primary_index"])
-        d.box[:] = [float(v) for v in a["box"]]
-        d.cutoff, d.kT = a["cutoff"], a["kT"]
-        d.max_translate, d.max_rotate = a["max_translate"], a["max_rotate"]
-        dp = lambda k: a[k].ctypes.data_as(C.POINTER(C.c_double))
-        ip = lambda k: a[k].ctypes.data_as(C.POINTER(C.c_int32))
-        d.x, d.y, d.z = dp("x"), dp("y"), dp("z")
-        d.sigma, d.epsilon, d.charge = dp("sigma"), dp("epsilon"), dp("charge")
-        d.mol_start, d.mol_len = ip("mol_start"), ip("mol_len")
-        d.mol_pidx_start, d.mol_pidx_count = ip("mol_pidx_start"), ip("mol_pidx_count")
-        d.primary_index = ip("primary_index")
-        return d
-
-
-def _kind(n_sites, rng, primaries, dummies=()):
-    """A rigid molecule: compact site geometry (1.1 A grid, first site at the
-    origin), OPLS-like parameters, a neutral charge pattern."""
-    g = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(2), indexing="ij"),
-                 -1).reshape(-1, 3)[:n_sites] * 1.1
-    sigma = rng.uniform(2.6, 3.6, n_sites)
-    eps = rng.uniform(0.03, 0.2, n_sites)
-    q = np.where(np.arange(n_sites) % 2 == 0, 0.3, -0.3)
-    if n_sites % 2:
-        q[-1] = 0.0
-    for k in dummies:
-        sigma[k] = -1.0
-    return dict(g=g, sigma=sigma, eps=eps, q=q, prim=list(primaries))
-
-
-def _rotation(rng):
-    qq = rng.normal(size=4)
-    w, x, y, z = qq / np.linalg.norm(qq)
-    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
-                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
-                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
-
-
-def build_box(kinds, n_mol, edge, cutoff, seed):
-    """Molecules cycle through `kinds`, one per jittered lattice point."""
-    rng = np.random.default_rng(seed)
-    k = int(np.ceil(n_mol ** (1 / 3)))
-    h = edge / k
-    pts = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3)
-    pts = pts[rng.permutation(len(pts))[:n_mol]] * h + 0.5 * h
-    xs, sg, ep, qs, ms, ml, ps, pc, pi = [], [], [], [], [], [], [], [], []
-    s0 = 0
-    for i in range(n_mol):
-        t = kinds[i % len(kinds)]
-        n = len(t["sigma"])
-        xyz = pts[i] + rng.uniform(-0.3, 0.3, 3) + (t["g"] - t["g"].mean(0)) @ _rotation(rng).T
-        xs.append(xyz)
-        sg.append(t["sigma"])
-        ep.append(t["eps"])
-        qs.append(t["q"])
-        ms.append(s0)
-        ml.append(n)
-        ps.append(len(pi))
-        pc.append(len(t["prim"]))
-        pi.extend(s0 + p for p in t["prim"])
-        s0 += n
-    xyz = np.concatenate(xs)
-    a = dict(x=np.ascontiguousarray(xyz[:, 0]), y=np.ascontiguousarray(xyz[:, 1]),
-             z=np.ascontiguousarray(xyz[:, 2]),
-             sigma=np.concatenate(sg), epsilon=np.concatenate(ep), charge=np.concatenate(qs),
-             mol_start=np.array(ms, np.int32), mol_len=np.array(ml, np.int32),
-             mol_pidx_start=np.array(ps, np.int32), mol_pidx_count=np.array(pc, np.int32),
-             primary_index=np.array(pi, np.int32),
-             box=np.array([edge] * 3), cutoff=cutoff, kT=0.5922, max_translate=0.4,
-             max_rotate=12.0)
-    return ArrayBox(a, mc_seed=seed + 1)
-
-
-def max_site_box(n_mol, edge, cutoff, seed):
-    rng = np.random.default_rng(100 + seed)
-    return build_box([_kind(32, rng, (3, 28), dummies=(5, 20))], n_mol, edge, cutoff, seed)
-
-
-def ragged_box(n_mol, edge, cutoff, seed):
-    rng = np.random.default_rng(200 + seed)
-    kinds = [_kind(1, rng, (0,)), _kind(32, rng, (0, 17, 31)), _kind(2, rng, (1,)),
-             _kind(5, rng, (2,), dummies=(4,))]
-    return build_box(kinds, n_mol, edge, cutoff, seed)
 
 
 # (builder, molecules, edge, cutoff): all-pairs boxes (fewer than 3 cells per
