@@ -277,10 +277,15 @@ __device__ __forceinline__ int block_scan_flag(bool f, int* lds, int& total) {
 
 // ------------------------------------------------------------ cell grid
 
+// Cell of a first-primary coordinate along one axis (n cells of width
+// 1/inv_w spanning the box edge).  Coordinates handed in through the ABI need
+// not lie in [0, L): a position within one box length outside falls in the
+// cell of its periodic image, so the 27 cells around it still hold every
+// candidate in range.  The final clamp keeps any other input on the grid.
 __device__ __forceinline__ int cell_coord(double p, double inv_w, int n) {
   int c = (int)floor(p * inv_w);
-  c = c < 0 ? 0 : c;
-  return c >= n ? n - 1 : c;
+  c = c < 0 ? c + n : (c >= n ? c - n : c);
+  return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
 __device__ __forceinline__ int cell_index(const CellGrid& g, int cx, int cy,

@@ -54,7 +54,7 @@ def kind(n_sites, rng, primaries, dummies=()):
     return dict(g=g, sigma=sigma, eps=eps, q=q, prim=list(primaries))
 
 
-def _rotation(rng):
+def rotation(rng):
     qq = rng.normal(size=4)
     w, x, y, z = qq / np.linalg.norm(qq)
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
@@ -74,7 +74,7 @@ def build_box(kinds, n_mol, edge, cutoff, seed):
     for i in range(n_mol):
         t = kinds[i % len(kinds)]
         n = len(t["sigma"])
-        xyz = pts[i] + rng.uniform(-0.3, 0.3, 3) + (t["g"] - t["g"].mean(0)) @ _rotation(rng).T
+        xyz = pts[i] + rng.uniform(-0.3, 0.3, 3) + (t["g"] - t["g"].mean(0)) @ rotation(rng).T
         xs.append(xyz)
         sg.append(t["sigma"])
         ep.append(t["eps"])

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import mcgpu_amd as m
-from edge_boxes import max_site_box, ragged_box
+from edge_boxes import max_site_box, ragged_box, rotation
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +93,84 @@ def test_edge_boxes_mixed_precision_error(name):
     print(f"{name} mixed: {k} moves aligned, max |dE err| {err.max():.3e}")
     assert k >= 300
     assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("name", sorted(BOXES))
+def test_edge_boxes_single_box_trial_moves(name):
+    """The single-box fused trial move (mcgpu_trial_move) on the edge boxes:
+    old and new molecule energies against the oracle's, the Metropolis
+    decision of SimulationStep.cpp:794-803 from those energies and the
+    caller's draw, and the commit on acceptance (the oracle's copy follows
+    the accepted moves, so later moves see the committed coordinates)."""
+    fn, n, edge, rc = BOXES[name]
+    ab = fn(n, edge, rc, seed=4)
+    ob = O.OracleBox(ab.arrays())
+    a = ob.a
+    db = m.DeviceBox(ab.desc())
+    if name.endswith("cells"):
+        db.neighbor(m.NEIGHBOR_CELL)
+    rng = np.random.default_rng(9)
+    n_acc = 0
+    for _ in range(150):
+        i = int(rng.integers(n))
+        s0, ln = int(a["mol_start"][i]), int(a["mol_len"][i])
+        old = np.stack([a[k][s0:s0 + ln].copy() for k in "xyz"])
+        pivot = old[:, :1]
+        new = pivot + rotation(rng) @ (old - pivot) + rng.uniform(-0.5, 0.5, (3, 1))
+        e_old = ob.molecule_energy(i)
+        for k, c in enumerate("xyz"):
+            a[c][s0:s0 + ln] = new[k]
+        e_new = ob.molecule_energy(i)
+        u = float(rng.random())
+        de = e_new - e_old
+        accept = de < 0 or np.exp(-de / a["kT"]) >= u
+        res = db.trial_move(i, new.reshape(-1), u)
+        assert res.old_energy == pytest.approx(e_old, rel=1e-12, abs=DE_TOL)
+        assert res.new_energy == pytest.approx(e_new, rel=1e-12, abs=DE_TOL)
+        assert bool(res.accepted) == accept
+        if not accept:
+            for k, c in enumerate("xyz"):
+                a[c][s0:s0 + ln] = old[k]
+        n_acc += accept
+    print(f"{name}: {n_acc} of 150 accepted")
+    assert 0 < n_acc < 150
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=DE_TOL)
+
+
+def test_first_primaries_outside_the_box_with_cells():
+    """Coordinates need not be wrapped into [0, L): molecules whose first
+    primary lies a few A past a face, in a box of 5 cells per edge.
+    Cell-list energies must equal the oracle's (all pairs, single wrap), and
+    the batched trace with cells must equal the oracle's trace."""
+    ab = max_site_box(512, 90.0, 10.0, seed=5)
+    a = ab.a
+    L = 90.0
+    # the whole box shifted by half a lattice spacing: the top x layer and the
+    # bottom y layer straddle the faces (pair displacements are unchanged)
+    a["x"] += 5.625
+    a["y"] -= 5.625
+    p = a["primary_index"][a["mol_pidx_start"]]
+    outside = (a["x"][p] >= L) | (a["x"][p] < 0) | (a["y"][p] >= L) | (a["y"][p] < 0)
+    assert outside.sum() >= 20
+    ob = O.OracleBox(ab.arrays())
+    db = m.DeviceBox(ab.desc())
+    db.neighbor(m.NEIGHBOR_CELL)
+    brute = m.DeviceBox(ab.desc())
+    for i in np.flatnonzero(outside)[:60]:
+        want = ob.molecule_energy(int(i))
+        assert db.molecule_energy(int(i)) == pytest.approx(want, rel=1e-12, abs=DE_TOL), i
+        assert brute.molecule_energy(int(i)) == pytest.approx(want, rel=1e-12, abs=DE_TOL), i
+    e_ref = ob.system_energy()
+    assert db.system_energy() == pytest.approx(e_ref, rel=1e-12, abs=DE_TOL)
+    assert brute.system_energy() == pytest.approx(e_ref, rel=1e-12, abs=DE_TOL)
+    assert db.system_pairs() == brute.system_pairs() > 0
+    steps = 1500
+    _, e0, _, want = ob.run_nvt(ab.mc_seed, steps, use_libc=False)
+    b = m.Batch(2)
+    s = b.add(ab, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+    b.run(steps)
+    tr = b.trace(s)
+    assert b.stats(s).start_energy == pytest.approx(e0, rel=1e-12, abs=DE_TOL)
+    assert np.array_equal(tr.mol, want["mol"])
+    assert np.array_equal(tr.accepted, want["accepted"])
+    assert de_ok(tr.de, want)
