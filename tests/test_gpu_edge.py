@@ -95,19 +95,24 @@ def test_edge_boxes_mixed_precision_error(name):
     assert err.max() <= 1e-4
 
 
-@pytest.mark.parametrize("name", sorted(BOXES))
-def test_edge_boxes_single_box_trial_moves(name):
+@pytest.mark.parametrize("name,nb", [(k, k.split("_")[1]) for k in sorted(BOXES)] +
+                         [(k, "verlet") for k in sorted(BOXES) if k.endswith("cells")])
+def test_edge_boxes_single_box_trial_moves(name, nb):
     """The single-box fused trial move (mcgpu_trial_move) on the edge boxes:
     old and new molecule energies against the oracle's, the Metropolis
     decision of SimulationStep.cpp:794-803 from those energies and the
     caller's draw, and the commit on acceptance (the oracle's copy follows
-    the accepted moves, so later moves see the committed coordinates)."""
+    the accepted moves, so later moves see the committed coordinates).  The
+    boxes of 3 cells per edge run with cell lists, and again with Verlet
+    lists (1 A skin, so lists go stale and are rebuilt along the way)."""
     fn, n, edge, rc = BOXES[name]
     ab = fn(n, edge, rc, seed=4)
     ob = O.OracleBox(ab.arrays())
     a = ob.a
     db = m.DeviceBox(ab.desc())
-    if name.endswith("cells"):
+    if nb == "verlet":
+        db.neighbor(m.NEIGHBOR_VERLET, 1.0)
+    elif nb == "cells":
         db.neighbor(m.NEIGHBOR_CELL)
     rng = np.random.default_rng(9)
     n_acc = 0
