@@ -63,10 +63,12 @@ def rotation(rng):
 
 
 def build_box(kinds, n_mol, edge, cutoff, seed):
-    """Molecules cycle through `kinds`, one per jittered lattice point."""
+    """Molecules cycle through `kinds`, one per jittered lattice point.
+    `edge`: one box edge, or three (a non-cubic box)."""
     rng = np.random.default_rng(seed)
     k = int(np.ceil(n_mol ** (1 / 3)))
-    h = edge / k
+    edges = np.broadcast_to(np.asarray(edge, dtype=np.float64), (3,)).copy()
+    h = edges / k
     pts = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3)
     pts = pts[rng.permutation(len(pts))[:n_mol]] * h + 0.5 * h
     xs, sg, ep, qs, ms, ml, ps, pc, pi = [], [], [], [], [], [], [], [], []
@@ -92,7 +94,7 @@ def build_box(kinds, n_mol, edge, cutoff, seed):
              mol_start=np.array(ms, np.int32), mol_len=np.array(ml, np.int32),
              mol_pidx_start=np.array(ps, np.int32), mol_pidx_count=np.array(pc, np.int32),
              primary_index=np.array(pi, np.int32),
-             box=np.array([edge] * 3), cutoff=cutoff, kT=0.5922, max_translate=0.4,
+             box=edges, cutoff=cutoff, kT=0.5922, max_translate=0.4,
              max_rotate=12.0)
     return ArrayBox(a, mc_seed=seed + 1)
 

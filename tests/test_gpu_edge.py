@@ -1,7 +1,7 @@
 """GPU parity at the edges of the input space the reference accepts:
 molecules of the maximum site count (32, MCGPU_MAX_MOL_SITES), several
-primary sites per molecule at a non-zero offset, dummy sites, and ragged
-boxes that mix 1-, 2-, 5- and 32-site molecules.  Boxes are built here from
+primary sites per molecule at a non-zero offset, dummy sites, ragged
+boxes that mix 1-, 2-, 5- and 32-site molecules, and non-cubic boxes.  Boxes are built here from
 plain arrays (the loader's descriptor layout), checked against the oracle's
 energies and against its trial-move traces for the same seed."""
 import numpy as np
@@ -23,6 +23,9 @@ BOXES = {
     "max32_cells": (max_site_box, 216, 56.0, 10.0),
     "ragged_brute": (ragged_box, 64, 26.0, 10.0),
     "ragged_cells": (ragged_box, 343, 62.0, 10.0),
+    # non-cubic: 3 x 3 x 4 and 3 x 4 x 3 cells
+    "noncubic32_cells": (max_site_box, 256, (56.0, 60.0, 75.0), 10.0),
+    "noncubicragged_cells": (ragged_box, 400, (62.0, 80.0, 66.0), 10.0),
 }
 
 
@@ -75,7 +78,7 @@ def test_edge_boxes_batch_trace_matches_oracle(name):
     assert de_ok(tr.de, want)
 
 
-@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells"])
+@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells", "noncubicragged_cells"])
 def test_edge_boxes_mixed_precision_error(name):
     fn, n, edge, rc = BOXES[name]
     ab = fn(n, edge, rc, seed=3)
@@ -181,7 +184,7 @@ def test_first_primaries_outside_the_box_with_cells():
     assert de_ok(tr.de, want)
 
 
-@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells"])
+@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells", "noncubic32_cells"])
 def test_edge_boxes_npt_energy_stays_consistent(name):
     """Batched NPT on the edge boxes with cell lists: the device-decided
     volume moves rescale by first primary, rebuild the cells and take a
