@@ -109,3 +109,12 @@ def ragged_box(n_mol, edge, cutoff, seed):
     kinds = [kind(1, rng, (0,)), kind(32, rng, (0, 17, 31)), kind(2, rng, (1,)),
              kind(5, rng, (2,), dummies=(4,))]
     return build_box(kinds, n_mol, edge, cutoff, seed)
+
+
+def small_box(n_mol, edge, cutoff, seed):
+    """Molecules of 1 to 5 sites (one kind with two primaries): the
+    full-energy kernel takes its half-shell path for them."""
+    rng = np.random.default_rng(300 + seed)
+    kinds = [kind(1, rng, (0,)), kind(4, rng, (0, 3)), kind(5, rng, (2,), dummies=(4,)),
+             kind(2, rng, (1,)), kind(3, rng, (0,))]
+    return build_box(kinds, n_mol, edge, cutoff, seed)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import mcgpu_amd as m
-from edge_boxes import max_site_box, ragged_box, rotation
+from edge_boxes import max_site_box, ragged_box, rotation, small_box
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +26,8 @@ BOXES = {
     # non-cubic: 3 x 3 x 4 and 3 x 4 x 3 cells
     "noncubic32_cells": (max_site_box, 256, (56.0, 60.0, 75.0), 10.0),
     "noncubicragged_cells": (ragged_box, 400, (62.0, 80.0, 66.0), 10.0),
+    # molecules of at most 5 sites (the half-shell full energy), 3 x 4 x 5 cells
+    "noncubicsmall_cells": (small_box, 600, (40.0, 52.0, 64.0), 9.0),
 }
 
 
@@ -184,7 +186,8 @@ def test_first_primaries_outside_the_box_with_cells():
     assert de_ok(tr.de, want)
 
 
-@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells", "noncubic32_cells"])
+@pytest.mark.parametrize("name", ["max32_cells", "ragged_cells", "noncubic32_cells",
+                                  "noncubicsmall_cells"])
 def test_edge_boxes_npt_energy_stays_consistent(name):
     """Batched NPT on the edge boxes with cell lists: the device-decided
     volume moves rescale by first primary, rebuild the cells and take a
