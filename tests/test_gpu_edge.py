@@ -213,3 +213,34 @@ def test_edge_boxes_npt_energy_stays_consistent(name):
     e = ob.system_energy() + ob.lrc_reference()
     print(f"{name}: box {st.box[0]:.4f} A, E {st.energy:.6f}")
     assert st.energy == pytest.approx(e, rel=1e-9, abs=1e-6)
+
+
+@pytest.mark.parametrize("nb", ["cells", "verlet"])
+def test_set_molecule_jumps_across_cells(nb):
+    """mcgpu_set_molecule_coords (the changeMolecule / rollback path) moving
+    molecules far across the box, into other cells and past the faces: the
+    neighbour structures must follow, so every molecule energy and the system
+    energy still equal the oracle's."""
+    fn, n, edge, rc = BOXES["noncubicragged_cells"]
+    ab = fn(n, edge, rc, seed=7)
+    ob = O.OracleBox(ab.arrays())
+    a = ob.a
+    L = np.asarray(a["box"])
+    db = m.DeviceBox(ab.desc())
+    if nb == "verlet":
+        db.neighbor(m.NEIGHBOR_VERLET, 1.0)
+    else:
+        db.neighbor(m.NEIGHBOR_CELL)
+    rng = np.random.default_rng(11)
+    for i in rng.choice(n, 24, replace=False):
+        s0, ln = int(a["mol_start"][i]), int(a["mol_len"][i])
+        shift = rng.uniform(-0.5, 0.5, 3) * L
+        shift[0] = 0.3 * L[0]     # along x: the top layers end up past the face
+        xyz = np.stack([a[c][s0:s0 + ln] for c in "xyz"]) + shift[:, None]
+        for k, c in enumerate("xyz"):
+            a[c][s0:s0 + ln] = xyz[k]
+        db.set_molecule(int(i), xyz.reshape(-1))
+    for i in range(n):
+        assert db.molecule_energy(i) == pytest.approx(ob.molecule_energy(i), rel=1e-12,
+                                                      abs=DE_TOL), i
+    assert db.system_energy() == pytest.approx(ob.system_energy(), rel=1e-12, abs=DE_TOL)
