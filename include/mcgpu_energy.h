@@ -110,7 +110,11 @@ int mcgpu_box_set_precision(mcgpu_box* box, int precision);
  * skin is used for MCGPU_NEIGHBOR_VERLET only (Angstrom). */
 int mcgpu_neighbor_rebuild(mcgpu_box* box, int kind, double skin);
 
-/* Whole-box coordinate transfer (GPUCopy::copyIn / copyOut). */
+/* Whole-box coordinate transfer (GPUCopy::copyIn / copyOut).  Coordinates
+ * handed in here, to mcgpu_set_molecule_coords or to mcgpu_trial_move need
+ * not be wrapped into [0, L): distances take the reference's single wrap
+ * (SimulationStep.cpp:377-385), and cell lists bin a molecule by the periodic
+ * image of its first primary (up to one box length outside). */
 int mcgpu_upload_coords(mcgpu_box* box, const double* x, const double* y,
                         const double* z);
 int mcgpu_download_coords(mcgpu_box* box, double* x, double* y, double* z);
