@@ -288,6 +288,47 @@ def precision_error_measure(moves: int = 3000):
     return out
 
 
+def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
+    """512 independent water boxes (64-4096 molecules, log-uniform; density
+    U[0.95, 1.05] g/cm3, T U[250, 400] K, rc 9 A) advanced together, every
+    box `sweeps` moves: whole-batch moves/s with FP32 pair terms and with
+    FP64, against the plain FP64 path on a sample of the same boxes."""
+    import mcgpu_amd as m
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for k, n in enumerate(log_uniform(rng, 64, 4096, n_boxes).astype(int)):
+        boxes.append(m.HostBox.generate(
+            m.WATER, int(n), density=float(rng.uniform(0.95, 1.05)), cutoff=9.0,
+            temperature=float(rng.uniform(250, 400)), max_translate=0.15,
+            max_rotate=15.0, seed=5000 + k))
+    out = {"boxes": n_boxes, "sweeps": sweeps,
+           "molecules_total": int(sum(b.desc().n_molecules for b in boxes))}
+    for name, prec in (("mixed", m.MIXED), ("fp64", m.FP64)):
+        b = m.Batch(n_boxes)
+        slots = [b.add(hb, steps=sweeps, seed=1 + i, precision=prec,
+                       neighbor=neighbor_for(hb)) for i, hb in enumerate(boxes)]
+        m.check(m.lib().mcgpu_device_synchronize())
+        t0 = time.perf_counter()
+        while not all(b.stats(s).done for s in slots):
+            b.run(sweeps)
+        dt = time.perf_counter() - t0
+        moves = sum(b.stats(s).steps_done for s in slots)
+        out[f"{name}_moves_per_s"] = moves / dt
+        out[f"{name}_seconds"] = dt
+        b.close()
+    sample = boxes[::32]
+    moves = t = 0.0
+    for i, hb in enumerate(sample):
+        t0 = time.perf_counter()
+        m.plain_run(hb, 2000, precision=m.FP64, neighbor=neighbor_for(hb), trace=False)
+        t += time.perf_counter() - t0
+        moves += 2000
+    out["plain_fp64_moves_per_s"] = moves / t
+    out["plain_sample"] = f"{len(sample)} of the boxes, 2000 moves each"
+    out["ratio_mixed_vs_plain"] = out["mixed_moves_per_s"] / out["plain_fp64_moves_per_s"]
+    return out
+
+
 def pmc_traffic():
     """HBM bytes per launch of the batch kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
@@ -313,6 +354,8 @@ def main() -> int:
                     help="measure only the full-energy kernel and print its JSON")
     ap.add_argument("--precision-only", action="store_true",
                     help="measure only the FP32-pair dE error and print its JSON")
+    ap.add_argument("--batch-512", action="store_true",
+                    help="measure only 512 water boxes advanced together and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -328,6 +371,9 @@ def main() -> int:
         return 0
     if args.precision_only:
         print(json.dumps(precision_error_measure()), flush=True)
+        return 0
+    if args.batch_512:
+        print(json.dumps(batch512_measure()), flush=True)
         return 0
 
     t_setup = time.perf_counter()
