@@ -25,6 +25,10 @@ batching server: value = trial moves / wall seconds.  Beside it:
   - the full-energy kernel's site-pair rate against its VALU roof, and the
     per-move dE error of FP32 pair terms against FP64 on a box of each
     configuration.
+Separate modes print one JSON object each and skip the stream:
+--full-energy-only, --precision-only, and --batch-512 (512 water boxes of
+64-4,096 molecules advanced together for 20,000 moves each, against the
+plain path).
 """
 from __future__ import annotations
 
