@@ -179,7 +179,9 @@ typedef struct mcgpu_run_params {
   int32_t volume_interval;/* NPT: a volume move every this many moves */
   int64_t steps;          /* trial moves to run for this box */
   double pressure;        /* NPT, kcal/mol/A^3 (1 atm = 1.4584e-5) */
-  double max_volume_delta;/* NPT, relative: dV = U(-1,1) * this * V */
+  double max_volume_delta;/* NPT, in (0, 0.5): volume step dV = U(-1,1) *
+                             this * V0, V0 the volume at admission (a fixed,
+                             symmetric step width) */
   double energy_offset;   /* added to the start energy (e.g. the LRC) */
   int32_t neighbor;       /* MCGPU_NEIGHBOR_BRUTE or MCGPU_NEIGHBOR_CELL */
   int32_t record_trace;   /* nonzero: keep per-move (mol, dE, accepted) */
@@ -227,6 +229,26 @@ int mcgpu_batch_download(mcgpu_batch* b, int32_t slot, double* x, double* y,
 int mcgpu_batch_trace(mcgpu_batch* b, int32_t slot, int32_t* mol,
                       double* delta, int8_t* accepted, int64_t cap,
                       int64_t* n);
+/* One NPT volume move as decided on the device (record_trace boxes): moves
+ * done before it, V, V' (product of the proposed edges; <= 0 for a proposal
+ * rejected outright), the running energy before it, E' = pairs at V' plus
+ * the rescaled LRC, the uniform u of the test, and the decision
+ * exp(N ln(V'/V) - (E' - E + P (V' - V)) / kT) >= u, which is
+ * SimCalcs::acceptVolumeMove (SimulationStep.cpp:476-488). */
+typedef struct mcgpu_volume_record {
+  int64_t step;
+  double v_old;
+  double v_new;
+  double e_old;
+  double e_new;
+  double u;
+  int32_t accepted;
+  int32_t reserved;
+} mcgpu_volume_record;
+/* Copies up to cap volume-move records of an NPT box added with
+ * record_trace (MCGPU_ERR_STATE otherwise); n = records copied. */
+int mcgpu_batch_volume_trace(mcgpu_batch* b, int32_t slot,
+                             mcgpu_volume_record* out, int64_t cap, int64_t* n);
 /* Per-launch time limit (microseconds, 0 = none): a box stops its sweeps
  * when its work-group has run this long, so small boxes do not wait idle
  * for the slowest one.  Results do not depend on it (each box's move

@@ -98,6 +98,13 @@ int mcgpu_config_read(const char* path, mcgpu_hostbox** out,
  * K-species form -- and the K-species form itself. */
 int mcgpu_lrc_reference(const mcgpu_box_desc* d, double* e);
 int mcgpu_lrc_species(const mcgpu_box_desc* d, double* e);
+/* The LRC a run starts from (its energy_offset): for MCGPU_NVT the
+ * reference's formula, so start and final energies match the reference's
+ * (Simulation.cpp:85); for MCGPU_NPT the K-species form, which volume moves
+ * rescale with 1/V and whose value enters their acceptance test (the
+ * reference's two-solvent formula counts N/2 molecules of molecules 0's and
+ * 1's kinds, wrong for any other mixture). */
+int mcgpu_lrc_for_run(const mcgpu_box_desc* d, int32_t ensemble, double* e);
 
 /* ---- plain driver: host RNG + one fused trial-move launch per move ---- */
 typedef struct mcgpu_plain_result {

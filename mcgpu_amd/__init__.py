@@ -70,6 +70,14 @@ class BoxStats(C.Structure):
     ]
 
 
+class VolumeRecord(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("v_old", C.c_double), ("v_new", C.c_double),
+        ("e_old", C.c_double), ("e_new", C.c_double), ("u", C.c_double),
+        ("accepted", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 class WorkloadParams(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("n_molecules", C.c_int32), ("n_second", C.c_int32),

@@ -7,12 +7,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import (BoxDesc, BoxStats, ConfigInfo, JobResult, MoveResult, PlainResult,
-               RunParams, ServerConfig, ServerStats, WorkloadParams, check,
-               lib)
+               RunParams, ServerConfig, ServerStats, VolumeRecord, WorkloadParams,
+               check, lib)
 
 __all__ = [
     "HostBox", "DeviceBox", "Batch", "Server", "plain_run", "strategy_run",
-    "workload", "WATER", "BICYCLIC", "MIXTURE",
+    "workload", "lrc_for_run", "WATER", "BICYCLIC", "MIXTURE",
 ]
 
 WATER, BICYCLIC, MIXTURE = 0, 1, 2
@@ -278,6 +278,15 @@ def strategy_run(box: HostBox, steps: int, trace: bool = True):
     return _run(lib().mcgpu_strategy_run, (box.handle,), steps, trace)
 
 
+def lrc_for_run(d: BoxDesc, ensemble: int = 0) -> float:
+    """The LRC a run starts from: the reference's formula under NVT, the
+    K-species form under NPT (mcgpu_lrc_for_run)."""
+    e = C.c_double()
+    check(lib().mcgpu_lrc_for_run(C.byref(d), C.c_int32(int(ensemble)), C.byref(e)),
+          "mcgpu_lrc_for_run")
+    return e.value
+
+
 class Batch:
     """Many independent boxes advanced by one persistent launch."""
 
@@ -305,9 +314,7 @@ class Batch:
             record_trace: bool = False) -> int:
         d = box.desc()
         if energy_offset is None:
-            e = C.c_double()
-            check(lib().mcgpu_lrc_reference(C.byref(d), C.byref(e)))
-            energy_offset = e.value
+            energy_offset = lrc_for_run(d, ensemble)
         p = RunParams(box.mc_seed if seed is None else seed, precision, ensemble,
                       volume_interval, steps, pressure, max_volume_delta,
                       energy_offset, neighbor, int(record_trace))
@@ -347,6 +354,24 @@ class Batch:
                                       C.c_int64(cap), C.byref(n)), "mcgpu_batch_trace")
         k = n.value
         return RunTrace(tm[:k], td[:k], ta[:k])
+
+    def volume_trace(self, slot: int) -> dict:
+        """The NPT volume moves of a record_trace box, as arrays: step,
+        v_old, v_new, e_old, e_new, u, accepted."""
+        steps = self._keep[slot][2]
+        cap = steps + 1
+        n = C.c_int64()
+        check(lib().mcgpu_batch_volume_trace(self._b, int(slot), None, C.c_int64(0),
+                                             C.byref(n)), "mcgpu_batch_volume_trace")
+        out = (VolumeRecord * max(cap, 1))()
+        check(lib().mcgpu_batch_volume_trace(self._b, int(slot), out, C.c_int64(cap),
+                                             C.byref(n)), "mcgpu_batch_volume_trace")
+        recs = out[:n.value]
+        cols = {k: np.array([getattr(r, k) for r in recs], dt) for k, dt in
+                (("step", np.int64), ("v_old", np.float64), ("v_new", np.float64),
+                 ("e_old", np.float64), ("e_new", np.float64), ("u", np.float64),
+                 ("accepted", np.int8))}
+        return cols
 
     def set_tick_budget(self, microseconds: float) -> None:
         check(lib().mcgpu_batch_set_tick_budget(self._b, C.c_double(microseconds)))
@@ -405,11 +430,9 @@ class Server:
                pressure: float = 0.0, max_volume_delta: float = 0.0,
                neighbor: int = 0, seed: int | None = None) -> int:
         d = box.desc()
-        e = C.c_double()
-        check(lib().mcgpu_lrc_reference(C.byref(d), C.byref(e)))
         p = RunParams(box.mc_seed if seed is None else seed, precision, ensemble,
-                      volume_interval, steps, pressure, max_volume_delta, e.value,
-                      neighbor, 0)
+                      volume_interval, steps, pressure, max_volume_delta,
+                      lrc_for_run(d, ensemble), neighbor, 0)
         jid = C.c_int64()
         self._boxes.append(box)   # the server reads it until run() returns
         check(lib().mcgpu_server_submit(self._s, box.handle, C.byref(p),

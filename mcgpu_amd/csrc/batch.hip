@@ -68,19 +68,21 @@ struct BatchBox {
   double* bx;       // [n_site] x3: coordinates before an NPT rescale
   double* by;
   double* bz;
+  mcgpu_volume_record* tr_vol;  // [tr_vol_cap] NPT volume moves (record_trace)
   double max_t;
   double max_r;
   int32_t precision;
   int32_t record;
-  // NPT: pressure (kcal/mol/A^3), relative volume step, moves between
-  // volume moves; cell geometry inputs (cutoff, primary spread, the cells
-  // the allocation holds; 0 = no cell lists)
+  // NPT: pressure (kcal/mol/A^3), half-width of the volume step (A^3),
+  // moves between volume moves; cell geometry inputs (cutoff, primary
+  // spread, the cells the allocation holds; 0 = no cell lists)
   double pressure;
-  double max_dv;
+  double dv;
   double cutoff;
   double spread;
   int32_t vol_interval;
   int32_t ncell_cap;
+  int64_t tr_vol_cap;
 };
 
 // One box of a batched full-energy / rescale / cell pass.
@@ -465,6 +467,9 @@ __device__ void set_geometry_dev(BatchBox& B) {
 // NPT volume move, first half, one thread per listed box (semantics: see
 // enqueue_volume_moves): u1 and u2 drawn from the box's own generator, the
 // new edges and cell geometry set, the scale factor handed to the rescale.
+// V' is the product of the new edges, as acceptVolumeMove reads it
+// (SimulationStep.cpp:476-481); a proposal <= 0 keeps the box (f = 1) and
+// is rejected by the decision.
 __global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -479,11 +484,12 @@ __global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k)
   BoxView& v = B.bv;
   for (int d = 0; d < 3; d++) c.L_old[d] = v.L[d];
   c.v_old = v.L[0] * v.L[1] * v.L[2];
-  c.v_new = c.v_old * (1.0 + B.max_dv * (2.0 * u1 - 1.0));
-  const double f = cbrt(c.v_new / c.v_old);
+  const double v_prop = c.v_old + B.dv * (2.0 * u1 - 1.0);
+  const double f = v_prop > 0.0 ? cbrt(v_prop / c.v_old) : 1.0;
   items[i].f = f;
   items[i].undo = 0;
   for (int d = 0; d < 3; d++) v.L[d] = c.L_old[d] * f;
+  c.v_new = v_prop > 0.0 ? v.L[0] * v.L[1] * v.L[2] : v_prop;
   set_geometry_dev(B);
 }
 
@@ -499,14 +505,32 @@ __global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k
   const int s = items[i].slot;
   BatchBox& B = table[s];
   BoxCtl& c = ctl[s];
-  const double lrc_new = c.lrc * c.v_old / c.v_new;
-  const double e_new = out[2 * i + 1] + lrc_new;
-  const double de = e_new - c.energy;
-  double prob = B.bv.n_mol * (log(c.v_new) - log(c.v_old));
-  prob -= (de + B.pressure * (c.v_new - c.v_old)) / B.bv.kT;
-  prob = exp_fp64(prob);
+  const bool valid = c.v_new > 0.0;
+  const double lrc_new = valid ? c.lrc * c.v_old / c.v_new : c.lrc;
+  const double e_new = valid ? out[2 * i + 1] + lrc_new : c.energy;
+  bool accept = false;
+  if (valid) {
+    // SimCalcs::acceptVolumeMove (SimulationStep.cpp:476-488)
+    const double de = e_new - c.energy;
+    double prob = B.bv.n_mol * (log(c.v_new) - log(c.v_old));
+    prob -= (de + B.pressure * (c.v_new - c.v_old)) / B.bv.kT;
+    prob = exp_fp64(prob);
+    accept = prob >= c.u_accept;
+  }
+  if (B.tr_vol && c.vol_moves < B.tr_vol_cap) {
+    mcgpu_volume_record r;
+    r.step = c.steps_done;
+    r.v_old = c.v_old;
+    r.v_new = c.v_new;
+    r.e_old = c.energy;
+    r.e_new = e_new;
+    r.u = c.u_accept;
+    r.accepted = accept ? 1 : 0;
+    r.reserved = 0;
+    B.tr_vol[c.vol_moves] = r;
+  }
   c.vol_moves++;
-  if (prob >= c.u_accept) {
+  if (accept) {
     c.vol_acc++;
     c.energy = e_new;
     c.lrc = lrc_new;
@@ -701,7 +725,8 @@ class Arena {
 struct BoxLayout {
   size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, meta, pidx, p64, p32;
   size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, partial, pairs,
-      total, bx, by, bz, trm, trd, tra;
+      total, bx, by, bz, trm, trd, tra, trv;
+  int64_t trv_cap;
   size_t bytes;
   int ncell, cap;
 };
@@ -763,6 +788,10 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
     L.trm = take(4 * (size_t)p.steps);
     L.trd = take(8 * (size_t)p.steps);
     L.tra = take((size_t)p.steps);
+    if (p.ensemble == MCGPU_NPT) {
+      L.trv_cap = p.steps / p.volume_interval + 1;
+      L.trv = take(sizeof(mcgpu_volume_record) * (size_t)L.trv_cap);
+    }
   }
   L.bytes = off;
   return L;
@@ -809,6 +838,7 @@ struct BatchSlot {
   int trial = -1;          // group being measured for the class (-1: none)
   int64_t steps_at_launch = 0;
   bool dirty = false;      // the device's control block is newer than h_ctl
+  bool vol_queued = false; // in the volume pass queued at the last boundary
 };
 
 // Host side of a batch.  Streams: `stream` carries the ticks (group 0 of the
@@ -826,6 +856,7 @@ struct mcgpu_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipEvent_t join[kGroups - 1] = {};
   hipEvent_t ev_ready = nullptr;  // set-up work queued so far is done
+  hipEvent_t ev_vol = nullptr;    // end of the last queued volume pass
   Arena arena;
   int32_t max_boxes = 0;
   std::vector<BatchSlot> slots;
@@ -957,11 +988,15 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 // decision, and the undo of rejected moves.  Semantics (the build's own;
 // the reference's NPT path, Ensemble.cpp:9-28 / SimulationStep.cpp:476-518,
 // always grows the box by 1.1 and never recomputes the energy):
-//   V' = V (1 + delta (2 u1 - 1)); molecules translated so first primaries
-//   scale by (V'/V)^(1/3); E' = E_pairs(V') + LRC(V'); dE = E' - E, with E
-//   the box's running energy (start energy plus accepted move dE);
-//   accept iff exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2, and then the
-//   running energy becomes E' exactly.
+//   V' = V + dv (2 u1 - 1), dv = delta V0 (V0 the volume at admission): a
+//   fixed-width, symmetric step, so acceptVolumeMove's test is the whole
+//   acceptance rule; molecules translated so first primaries scale by
+//   f = (V'/V)^(1/3); V' then taken as the product of the new edges;
+//   E' = E_pairs(V') + LRC V/V'; dE = E' - E, with E the box's running
+//   energy (start energy plus accepted move dE); accept iff
+//   exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2 (SimulationStep.cpp:
+//   476-488), and then the running energy becomes E' exactly.  A proposal
+//   V' <= 0 is rejected.
 int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   int rc;
   const int K = (int)pend.size();
@@ -995,7 +1030,11 @@ int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
   MCGPU_HIP_TRY(hipGetLastError());
   // cells again for the restored boxes (a rebuild of an accepted box's
   // lists reproduces them: same positions and geometry, ordered slots)
-  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st);
+  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st))) return rc;
+  // a removal of one of these boxes before the pass ends waits for it
+  MCGPU_HIP_TRY(hipEventRecord(b->ev_vol, st));
+  for (int k = 0; k < K; k++) b->slots[pend[k]].vol_queued = true;
+  return MCGPU_OK;
 }
 
 // Work-group-size tuner.  Per box class (neighbour source, pair precision,
@@ -1050,6 +1089,8 @@ int batch_boundary(mcgpu_batch* b) {
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   b->t_wait += host_now() - t0;
   b->pending = false;
+  // every volume pass queued before this tick has ended with it
+  for (BatchSlot& S : b->slots) S.vol_queued = false;
   for (int s : b->launched) {
     b->h_ctl[s] = b->h_ctl_rb[s];
     b->h_rng[s] = b->h_rng_rb[s];
@@ -1148,7 +1189,8 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
             hipStreamCreateWithFlags(&b->sstream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
             hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_vol, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; ok && g < kGroups - 1; g++)
     ok = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming) == hipSuccess;
@@ -1212,7 +1254,7 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   for (void* p : {(void*)b->h_table, (void*)b->h_ctl, (void*)b->h_ctl_rb, (void*)b->h_rng,
                   (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_items})
     if (p) (void)hipHostFree(p);
-  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_ready})
+  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_ready, b->ev_vol})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->join)
     if (e) (void)hipEventDestroy(e);
@@ -1360,7 +1402,9 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.by = (double*)(base + L.by);
   E.bz = (double*)(base + L.bz);
   E.pressure = p->pressure;
-  E.max_dv = p->max_volume_delta;
+  // fixed-width volume step: a symmetric proposal, so the acceptance test
+  // needs no Hastings factor
+  E.dv = p->max_volume_delta * (d->box[0] * d->box[1] * d->box[2]);
   E.cutoff = d->cutoff;
   E.spread = pt.spread;
   E.vol_interval = p->volume_interval;
@@ -1368,6 +1412,8 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.tr_mol = (int32_t*)(base + L.trm);
   E.tr_de = (double*)(base + L.trd);
   E.tr_acc = (int8_t*)(base + L.tra);
+  E.tr_vol = L.trv_cap > 0 ? (mcgpu_volume_record*)(base + L.trv) : nullptr;
+  E.tr_vol_cap = L.trv_cap;
   rng31_seed(b->h_rng[s], p->seed);
   const hipStream_t st = b->sstream;
   const int32_t zero[3] = {0, 0, 0};
@@ -1450,6 +1496,11 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
   int rc = batch_boundary(b);
   if (rc) return rc;
   if (b->slots[s].dirty) MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
+  // a box at its volume stop has a volume pass queued on the tick stream
+  // that still writes its control block and arena block: the set-up stream,
+  // which carries this slot's reset and every later upload into the freed
+  // block, waits for that pass
+  if (b->slots[s].vol_queued) MCGPU_HIP_TRY(hipStreamWaitEvent(b->sstream, b->ev_vol, 0));
   b->h_ctl[s] = BoxCtl{};
   b->h_ctl[s].status = kDone;
   if ((rc = upload_ctl(b, s, b->sstream))) return rc;
@@ -1620,6 +1671,31 @@ extern "C" int mcgpu_batch_trace(mcgpu_batch* b, int32_t s, int32_t* mol,
     if (mol) MCGPU_HIP_TRY(hipMemcpyAsync(mol, E.tr_mol, 4 * k, hipMemcpyDeviceToHost, b->stream));
     if (delta) MCGPU_HIP_TRY(hipMemcpyAsync(delta, E.tr_de, 8 * k, hipMemcpyDeviceToHost, b->stream));
     if (acc) MCGPU_HIP_TRY(hipMemcpyAsync(acc, E.tr_acc, k, hipMemcpyDeviceToHost, b->stream));
+    MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  }
+  *n = k;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_volume_trace(mcgpu_batch* b, int32_t s,
+                                        mcgpu_volume_record* out, int64_t cap,
+                                        int64_t* n) {
+  if (!b || s < 0 || s >= b->max_boxes || !b->slots[s].used || !n) {
+    set_error("mcgpu_batch_volume_trace: bad argument");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_quiesce(b);
+  if (rc) return rc;
+  const BatchBox& E = b->h_table[s];
+  if (!E.tr_vol) {
+    set_error("mcgpu_batch_volume_trace: box is not NPT or was added without record_trace");
+    return MCGPU_ERR_STATE;
+  }
+  const int64_t k = std::min<int64_t>(std::min<int64_t>(cap, E.tr_vol_cap),
+                                      b->h_ctl[s].vol_moves);
+  if (k > 0 && out) {
+    MCGPU_HIP_TRY(hipMemcpyAsync(out, E.tr_vol, sizeof(mcgpu_volume_record) * k,
+                                 hipMemcpyDeviceToHost, b->stream));
     MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   }
   *n = k;

@@ -181,3 +181,9 @@ extern "C" int mcgpu_lrc_species(const mcgpu_box_desc* d, double* e) {
   *e = mcgpu::lrc_species(*d);
   return MCGPU_OK;
 }
+
+extern "C" int mcgpu_lrc_for_run(const mcgpu_box_desc* d, int32_t ensemble, double* e) {
+  if (!d || !e || (ensemble != MCGPU_NVT && ensemble != MCGPU_NPT)) return MCGPU_ERR_INVALID;
+  *e = ensemble == MCGPU_NPT ? mcgpu::lrc_species(*d) : mcgpu::lrc_reference(*d);
+  return MCGPU_OK;
+}

@@ -123,8 +123,8 @@ int parse_args(int argc, char** argv, Args& a) {
         break;
       case kLongVolume:
         a.max_volume_delta = std::atof(optarg);
-        if (!(a.max_volume_delta > 0 && a.max_volume_delta < 1)) {
-          std::fprintf(stderr, "%s: --max-volume-delta must lie in (0, 1)\n", kApp);
+        if (!(a.max_volume_delta > 0 && a.max_volume_delta < 0.5)) {
+          std::fprintf(stderr, "%s: --max-volume-delta must lie in (0, 0.5)\n", kApp);
           return -1;
         }
         break;
@@ -306,7 +306,7 @@ int main(int argc, char** argv) {
   mcgpu_box_desc desc{};
   mcgpu_hostbox_desc(hb, &desc);
   double lrc = 0.0;
-  mcgpu_lrc_reference(&desc, &lrc);
+  mcgpu_lrc_for_run(&desc, npt ? MCGPU_NPT : MCGPU_NVT, &lrc);
   std::printf("Using seed: %d\n", (int)seed);
 
   // ---- device

@@ -293,11 +293,64 @@ static int32_t draw(orc_rng* g, int use_libc) {
   return use_libc ? (int32_t)rand() : orc_glibc_rand(g);
 }
 
-/* Simulation.cpp:65-264 (NVT, BruteForce, serial).  The molecule index is
- * clamped to n_mol-1 for the 2^-31-probability draw rand() == RAND_MAX, where
- * the reference indexes one past its arrays (SimulationStep.cpp:25-27). */
-double orc_run_nvt(orc_box* b, uint32_t seed, int64_t steps, int use_libc,
-                   orc_trace* t, double* e0_out, int64_t* n_acc) {
+/* One NPT volume move (the build's own semantics; the reference's NPT path,
+ * Ensemble.cpp:9-28, always grows the box by 1.1 and never recomputes the
+ * energy).  u1, u2 ~ U(0,1) from the MC stream; V' = V + dv (2 u1 - 1), a
+ * fixed-width (symmetric) step; molecules translated so first primaries
+ * scale by f = (V'/V)^(1/3); the new volume is the product of the new edges,
+ * as acceptVolumeMove reads it (SimulationStep.cpp:476-481); the LRC scales
+ * as 1/V; accept iff acceptVolumeMove's probability (SimulationStep.cpp:
+ * 482-487) >= u2.  A proposal V' <= 0 is rejected without a rescale. */
+static int volume_move(orc_box* b, orc_rng* g, int use_libc, const orc_npt* p,
+                       double* energy, double* lrc, double rec[4]) {
+  const double u1 = random_real_from(draw(g, use_libc), 0.0, 1.0);
+  const double u2 = random_real_from(draw(g, use_libc), 0.0, 1.0);
+  const double v_old = b->L[0] * b->L[1] * b->L[2];
+  const double v_prop = v_old + p->dv * (2.0 * u1 - 1.0);
+  double v_new = v_prop, e_new = *energy;
+  int accept = 0;
+  if (v_prop > 0) {
+    const size_t n = (size_t)b->n_site;
+    double* bk = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    double L_old[3] = {b->L[0], b->L[1], b->L[2]};
+    memcpy(bk, b->x, sizeof(double) * n);
+    memcpy(bk + n, b->y, sizeof(double) * n);
+    memcpy(bk + 2 * n, b->z, sizeof(double) * n);
+    orc_volume_scale(b, cbrt(v_prop / v_old));
+    v_new = b->L[0] * b->L[1] * b->L[2];
+    const double lrc_new = *lrc * v_old / v_new;
+    e_new = orc_system_energy(b) + lrc_new;
+    const double prob = orc_volume_accept_prob(b->n_mol, v_old, v_new, e_new - *energy,
+                                               p->pressure, b->kT);
+    accept = prob >= u2;
+    if (accept) {
+      *energy = e_new;
+      *lrc = lrc_new;
+    } else {
+      memcpy(b->x, bk, sizeof(double) * n);
+      memcpy(b->y, bk + n, sizeof(double) * n);
+      memcpy(b->z, bk + 2 * n, sizeof(double) * n);
+      for (int d = 0; d < 3; d++) b->L[d] = L_old[d];
+    }
+    free(bk);
+  }
+  if (rec) {
+    rec[0] = v_old;
+    rec[1] = v_new;
+    rec[2] = e_new;
+    rec[3] = u2;
+  }
+  return accept;
+}
+
+/* Simulation.cpp:65-264 (BruteForce, serial), NVT or, with npt, the build's
+ * NPT loop: a volume move after every volume_interval moves while moves
+ * remain.  The molecule index is clamped to n_mol-1 for the
+ * 2^-31-probability draw rand() == RAND_MAX, where the reference indexes one
+ * past its arrays (SimulationStep.cpp:25-27). */
+static double run_loop(orc_box* b, uint32_t seed, int64_t steps, int use_libc,
+                       const orc_npt* npt, orc_trace* t, orc_vtrace* vt,
+                       double* e0_out, int64_t* n_acc) {
   orc_rng g;
   int32_t maxlen = 0;
   for (int32_t m = 0; m < b->n_mol; m++) {
@@ -308,11 +361,13 @@ double orc_run_nvt(orc_box* b, uint32_t seed, int64_t steps, int use_libc,
     srand(seed);
   else
     orc_glibc_srand(&g, seed);
-  const double lrc = orc_lrc_reference(b);
+  /* NVT: the reference's LRC (Simulation.cpp:85); NPT: the K-species form,
+   * which the volume moves rescale */
+  double lrc = npt ? orc_lrc_species(b) : orc_lrc_reference(b);
   double energy = orc_system_energy(b) + lrc;
   if (e0_out) *e0_out = energy;
   double* rb = (double*)malloc(sizeof(double) * 3 * (size_t)(maxlen > 0 ? maxlen : 1));
-  int64_t acc_count = 0;
+  int64_t acc_count = 0, nv = 0;
   for (int64_t s = 0; s < steps; s++) {
     int32_t idx = (int32_t)random_real_from(draw(&g, use_libc), 0, b->n_mol);
     if (idx >= b->n_mol) idx = b->n_mol - 1;
@@ -352,10 +407,37 @@ double orc_run_nvt(orc_box* b, uint32_t seed, int64_t steps, int use_libc,
       if (t->de) t->de[s] = de;
       if (t->accepted) t->accepted[s] = (int8_t)accept;
     }
+    if (npt && (s + 1) % npt->volume_interval == 0 && s + 1 < steps) {
+      double rec[4];
+      const double e_before = energy;
+      const int va = volume_move(b, &g, use_libc, npt, &energy, &lrc, rec);
+      if (vt && nv < vt->cap) {
+        if (vt->step) vt->step[nv] = s + 1;
+        if (vt->v_old) vt->v_old[nv] = rec[0];
+        if (vt->v_new) vt->v_new[nv] = rec[1];
+        if (vt->e_old) vt->e_old[nv] = e_before;
+        if (vt->e_new) vt->e_new[nv] = rec[2];
+        if (vt->u) vt->u[nv] = rec[3];
+        if (vt->accepted) vt->accepted[nv] = (int8_t)va;
+      }
+      nv++;
+    }
   }
   free(rb);
+  if (vt) vt->n = nv;
   if (n_acc) *n_acc = acc_count;
   return energy;
+}
+
+double orc_run_nvt(orc_box* b, uint32_t seed, int64_t steps, int use_libc,
+                   orc_trace* t, double* e0_out, int64_t* n_acc) {
+  return run_loop(b, seed, steps, use_libc, NULL, t, NULL, e0_out, n_acc);
+}
+
+double orc_run_npt(orc_box* b, uint32_t seed, int64_t steps, const orc_npt* p,
+                   int use_libc, orc_trace* t, orc_vtrace* vt, double* e0_out,
+                   int64_t* n_acc) {
+  return run_loop(b, seed, steps, use_libc, p, t, vt, e0_out, n_acc);
 }
 
 int64_t orc_inrange_pairs(const orc_box* b, int32_t* out, int64_t cap) {

@@ -23,11 +23,21 @@
  *   orc_run_nvt            the reference trace: identical molecule choice,
  *                          dE and accept/reject for every move.
  *   orc_lrc_reference      the textbook single-species tail correction.
- *   orc_lrc_species,       parity unpinned by the reference (its NPT path is
- *   orc_volume_scale,      defective); pinned by closed forms: a scale by f
- *   orc_volume_accept      preserves internal geometry and multiplies primary
- *                          positions by f; the LRC scales as 1/V; the
- *                          acceptance formula is checked term by term.
+ *   orc_lrc_species        the textbook single-species value for two species
+ *                          with identical LJ sites (N_A + N_B molecules), and
+ *                          a numerical quadrature of the LJ tail integral,
+ *                          with orc_lj_params as the pair potential, for a
+ *                          two-species box of different sites.
+ *   orc_volume_accept_prob the reference's own SimCalcs::acceptVolumeMove
+ *                          (oracle/_ref/ref_volume) on a grid of (N, V, V',
+ *                          dE, P, kT, seed): identical decisions; V' = V
+ *                          gives exp(-dE/kT); falls as P grows for V' > V.
+ *   orc_volume_scale       closed forms: internal geometry preserved,
+ *                          primary positions multiplied by f.
+ *   orc_run_npt            the build's own NPT semantics (the reference's
+ *                          volume move is defective): its decisions use the
+ *                          pinned functions above; the GPU path must give
+ *                          the same move and volume-move sequence.
  */
 #ifndef MCGPU_ORACLE_H
 #define MCGPU_ORACLE_H
@@ -131,6 +141,35 @@ double orc_lrc_species(const orc_box* b);
  * exp(N ln(V'/V) - (dE + P dV)/kT) */
 double orc_volume_accept_prob(int32_t n_mol, double v_old, double v_new,
                               double de, double pressure, double kT);
+
+/* The build's NPT loop: the NVT loop above with a volume move after every
+ * volume_interval moves while moves remain.  Volume move: u1, u2 drawn from
+ * the MC stream; V' = V + dv (2 u1 - 1) (fixed width dv, in A^3); molecules
+ * translated so first primaries scale by (V'/V)^(1/3); V' taken as the
+ * product of the new edges; LRC (K-species form) scaled as 1/V; accept iff
+ * orc_volume_accept_prob(N, V, V', E' - E, P, kT) >= u2, with E the running
+ * energy; a proposal V' <= 0 is rejected. */
+typedef struct orc_npt {
+  int32_t volume_interval;
+  int32_t reserved;
+  double pressure; /* kcal/mol/A^3 */
+  double dv;       /* half-width of the volume step, A^3 */
+} orc_npt;
+
+/* Per volume move: moves done before it, V, V', running energy before it,
+ * E' at V' (pairs + LRC), u2, accepted.  Arrays of length cap (NULL to
+ * skip); n receives the number of volume moves made. */
+typedef struct orc_vtrace {
+  int64_t cap;
+  int64_t n;
+  int64_t* step;
+  double *v_old, *v_new, *e_old, *e_new, *u;
+  int8_t* accepted;
+} orc_vtrace;
+
+double orc_run_npt(orc_box* b, uint32_t seed, int64_t steps, const orc_npt* p,
+                   int use_libc, orc_trace* t, orc_vtrace* vt, double* e0_out,
+                   int64_t* n_acc);
 
 #ifdef __cplusplus
 }

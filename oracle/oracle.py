@@ -14,6 +14,7 @@ import numpy as np
 _DIR = Path(__file__).resolve().parent
 _SO = _DIR / "bin" / "liboracle.so"
 REF_TRACE = _DIR / "_ref" / "ref_trace"
+REF_VOLUME = _DIR / "_ref" / "ref_volume"
 _lib = None
 
 _d = C.POINTER(C.c_double)
@@ -41,6 +42,17 @@ class _Trace(C.Structure):
                 ("de", _d), ("accepted", C.POINTER(C.c_int8))]
 
 
+class _Npt(C.Structure):
+    _fields_ = [("volume_interval", C.c_int32), ("reserved", C.c_int32),
+                ("pressure", C.c_double), ("dv", C.c_double)]
+
+
+class _VTrace(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("n", C.c_int64), ("step", C.POINTER(C.c_int64)),
+                ("v_old", _d), ("v_new", _d), ("e_old", _d), ("e_new", _d), ("u", _d),
+                ("accepted", C.POINTER(C.c_int8))]
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -51,7 +63,7 @@ def lib() -> C.CDLL:
                      "orc_lj_params", "orc_coulomb_params", "orc_pair_sum",
                      "orc_molecule_energy", "orc_system_energy",
                      "orc_lrc_reference", "orc_lrc_species", "orc_run_nvt",
-                     "orc_random_real", "orc_volume_accept_prob"]:
+                     "orc_random_real", "orc_volume_accept_prob", "orc_run_npt"]:
             getattr(_lib, name).restype = C.c_double
         _lib.orc_lj_params.argtypes = [C.c_double] * 5
         _lib.orc_coulomb_params.argtypes = [C.c_double] * 3
@@ -143,6 +155,14 @@ class OracleBox:
 
     def run_nvt(self, seed: int, steps: int, use_libc: bool = True):
         """The reference loop; returns (final_E, E0, n_acc, trace dict)."""
+        t, tr = self._trace(steps)
+        e0 = C.c_double()
+        nacc = C.c_int64()
+        fin = lib().orc_run_nvt(self.ref, C.c_uint32(seed), C.c_int64(steps),
+                                int(use_libc), C.byref(tr), C.byref(e0), C.byref(nacc))
+        return fin, e0.value, nacc.value, t
+
+    def _trace(self, steps):
         t = {k: np.zeros(steps, dt) for k, dt in
              (("mol", np.int32), ("vertex", np.int32), ("old_e", np.float64),
               ("new_e", np.float64), ("de", np.float64), ("accepted", np.int8))}
@@ -150,11 +170,44 @@ class OracleBox:
                     t["old_e"].ctypes.data_as(_d), t["new_e"].ctypes.data_as(_d),
                     t["de"].ctypes.data_as(_d),
                     t["accepted"].ctypes.data_as(C.POINTER(C.c_int8)))
+        return t, tr
+
+    def run_npt(self, seed: int, steps: int, volume_interval: int, pressure: float,
+                dv: float, use_libc: bool = False):
+        """The build's NPT loop (volume move after every volume_interval
+        moves, fixed-width step dv in A^3).  Returns (final_E, E0, n_acc,
+        move trace dict, volume-move trace dict)."""
+        t, tr = self._trace(steps)
+        cap = steps // max(volume_interval, 1) + 1
+        v = {k: np.zeros(cap, dt) for k, dt in
+             (("step", np.int64), ("v_old", np.float64), ("v_new", np.float64),
+              ("e_old", np.float64), ("e_new", np.float64), ("u", np.float64),
+              ("accepted", np.int8))}
+        vt = _VTrace(cap, 0, v["step"].ctypes.data_as(C.POINTER(C.c_int64)),
+                     *(v[k].ctypes.data_as(_d) for k in ("v_old", "v_new", "e_old",
+                                                          "e_new", "u")),
+                     v["accepted"].ctypes.data_as(C.POINTER(C.c_int8)))
+        p = _Npt(int(volume_interval), 0, float(pressure), float(dv))
         e0 = C.c_double()
         nacc = C.c_int64()
-        fin = lib().orc_run_nvt(self.ref, C.c_uint32(seed), C.c_int64(steps),
-                                int(use_libc), C.byref(tr), C.byref(e0), C.byref(nacc))
-        return fin, e0.value, nacc.value, t
+        fin = lib().orc_run_npt(self.ref, C.c_uint32(seed), C.c_int64(steps), C.byref(p),
+                                int(use_libc), C.byref(tr), C.byref(vt), C.byref(e0),
+                                C.byref(nacc))
+        n = int(vt.n)
+        return fin, e0.value, nacc.value, t, {k: a[:n] for k, a in v.items()}
+
+
+def reference_volume_decisions(state_path, rows):
+    """The reference's own SimCalcs::acceptVolumeMove (oracle/_ref/ref_volume)
+    on rows of (seed, n_molecules, kT, v_old, Lx, Ly, Lz, dE, pressure);
+    returns its decisions (0/1), or None when the reference build is absent."""
+    if not REF_VOLUME.exists():
+        return None
+    text = "".join("%d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n" % tuple(r)
+                   for r in rows)
+    out = subprocess.run([str(REF_VOLUME), str(state_path)], input=text,
+                         capture_output=True, text=True, check=True).stdout
+    return np.array([int(x) for x in out.split()], np.int8)
 
 
 class GlibcRng:

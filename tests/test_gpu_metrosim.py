@@ -89,6 +89,7 @@ def test_metrosim_npt_run(tmp_path):
     import ctypes as C
     d = hb.desc()
     lrc = C.c_double()
-    m.check(m.lib().mcgpu_lrc_reference(C.byref(d), C.byref(lrc)))
+    # NPT runs carry the K-species LRC
+    m.check(m.lib().mcgpu_lrc_species(C.byref(d), C.byref(lrc)))
     final = float(re.search(r"Final Energy: (\S+)", r.stdout).group(1))
     assert abs(db.system_energy() + lrc.value - final) < 2e-3

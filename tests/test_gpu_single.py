@@ -142,9 +142,9 @@ def test_neighbor_paths_keep_trace_parity(kind, skin):
 
 
 def test_mixed_precision_error_bound():
-    """FP32 LJ terms and FP32-estimated 1/r, FP64 sums: per-move
-    |dE - dE_fp64| over a trajectory where both make the same decisions,
-    within 1e-4 kcal/mol on every move."""
+    """FP32 LJ terms and FP32-estimated 1/r, FP64 sums: the molecule choices
+    and accept/reject decisions of all 10,000 config-0 moves equal the
+    reference's, and per-move |dE - dE_ref| stays within 1e-4 kcal/mol."""
     ref = np.load(GOLDEN / "water256.ref.npz")
     steps = 10000
     res, tr = m.plain_run(load("water256"), steps, precision=m.MIXED)
@@ -153,7 +153,9 @@ def test_mixed_precision_error_bound():
     err = np.abs(tr.de[:n] - ref["de"][:n])
     print(f"mixed water256: {n} moves aligned, max |dE err| {err.max():.3e}, "
           f"p99 {np.quantile(err, 0.99):.3e}")
-    assert n >= 1000
+    # accept/reject identical to the reference on every move of config 0
+    assert n == steps
+    assert np.array_equal(tr.mol, ref["mol"][:steps])
     assert err.max() <= 1e-4
 
 
