@@ -10,9 +10,12 @@ and half NPT (1 atm, a volume move every 500 steps), half FP64 and half
 FP32-pair arithmetic.  Molecule counts are drawn from 24 log-spaced classes
 so every class's box is generated once; each job has its own MC seed.
 
-A step is one job of the stream.  The timed region serves exactly K jobs
-(arrival of the first to completion of the last) through the continuous-
-batching server: value = trial moves / wall seconds.  Beside it:
+A step is 100 jobs of one continuous stream: the warm-up takes the first
+W x 100 jobs (at most 2000 moves each, all arriving at once) and the timed
+region the next K x 100 (K = 20 covers 2,000 jobs, ~44 M moves), served
+through the continuous-batching server from the first arrival to the last
+completion: value = trial moves / wall seconds, ms_per_step = wall / K.
+Beside it:
   - the plain FP64 GPU path (host RNG, one fused trial-move launch per
     move) on a sample of the same stream's jobs, and the ratio;
   - p50/p99 job latency (completion - arrival);
@@ -22,9 +25,13 @@ batching server: value = trial moves / wall seconds.  Beside it:
   - the reference's own CPU path (oracle/_ref) timed on host cores on a
     bounded sample of the same jobs (kind "reference"), or the oracle
     restatement when the reference build is absent (kind "port");
+  - the same with one reference process per host core (up to 16), and the
+    host CPU's model;
   - the full-energy kernel's site-pair rate against its VALU roof, and the
     per-move dE error of FP32 pair terms against FP64 on a box of each
-    configuration.
+    configuration;
+  - 512 water boxes of 64-4,096 molecules advanced together for 20,000
+    moves each (key "batch512").
 Separate modes print one JSON object each and skip the stream:
 --full-energy-only, --precision-only, and --batch-512 (512 water boxes of
 64-4,096 molecules advanced together for 20,000 moves each, against the
@@ -53,6 +60,7 @@ METRIC = ("MC trial moves/s, batched FP32/FP64-mixed vs plain FP64 on 1 GPU; "
 HBM_PEAK_GBS = 8000.0
 LAMBDA = 200.0
 N_CLASSES = 24
+JOBS_PER_STEP = 100
 
 
 def log_uniform(rng, lo, hi, size=None):
@@ -60,15 +68,18 @@ def log_uniform(rng, lo, hi, size=None):
 
 
 def make_stream(n_jobs: int, seed: int = 2024):
-    """The job stream: arrival times and per-job parameters (seeded)."""
+    """The job stream: arrival times and per-job parameters (seeded).  Each
+    job's values are drawn in turn, so a longer stream extends a shorter
+    one: make_stream(n)[:k] == make_stream(k)."""
     import mcgpu_amd as m
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / LAMBDA, n_jobs))
-    arrivals -= arrivals[0]
     classes = np.unique(np.round(np.exp(np.linspace(math.log(128), math.log(16384),
                                                      N_CLASSES))).astype(int))
     jobs = []
+    t = 0.0
     for k in range(n_jobs):
+        if k:
+            t += float(rng.exponential(1.0 / LAMBDA))
         kind = m.WATER if rng.random() < 0.5 else m.BICYCLIC
         n = int(log_uniform(rng, 128, 16384))
         n = int(classes[np.argmin(np.abs(np.log(classes) - math.log(n)))])
@@ -76,8 +87,7 @@ def make_stream(n_jobs: int, seed: int = 2024):
         npt = rng.random() < 0.5
         mixed = rng.random() < 0.5
         jobs.append(dict(kind=kind, n=n, steps=steps, npt=npt, mixed=mixed,
-                         mc_seed=int(rng.integers(1, 2**31 - 1)),
-                         arrival=float(arrivals[k])))
+                         mc_seed=int(rng.integers(1, 2**31 - 1)), arrival=t))
     return jobs
 
 
@@ -198,6 +208,50 @@ def cpu_baseline(jobs, pool, budget_s: float):
                         f"single thread, {secs:.1f} s of loop time"))
 
 
+def cpu_baseline_all_cores(jobs, pool, budget_s: float, max_cores: int = 16):
+    """The reference's CPU path with one job per core, all at once (jobs are
+    independent boxes): the first `cores` jobs of the stream, each for
+    about budget_s / 2 seconds of loop; aggregate = sum of per-process move
+    rates while all run together."""
+    ref = ROOT / "oracle" / "_ref" / "ref_trace"
+    if not ref.exists():
+        return None
+    cores = max(1, min(max_cores, os.cpu_count() or 1))
+    per_job = max(1.0, budget_s / 2)
+    procs = []
+    with tempfile.TemporaryDirectory() as td:
+        for k, j in enumerate(jobs[:cores]):
+            box = pool.get(j["kind"], j["n"])
+            steps = int(max(20, min(j["steps"], 4e6 / j["n"] * per_job / 2)))
+            p = Path(td) / f"b{k}.state"
+            box.write_state(str(p))
+            procs.append((steps, subprocess.Popen([str(ref), str(p), str(steps), "-"],
+                                                  stdout=subprocess.PIPE, text=True)))
+        rate, moves = 0.0, 0
+        for steps, pr in procs:
+            out, _ = pr.communicate()
+            if pr.returncode != 0:
+                return None
+            loop = float(re.search(r"Loop Seconds: ([0-9.eE+-]+)", out).group(1))
+            rate += steps / loop
+            moves += steps
+    return dict(value=rate, unit="moves/s", cores=len(procs), kind="reference",
+                sample=(f"first {len(procs)} jobs of the stream, one process each, "
+                        f"{moves} moves in total (each capped to ~{per_job:.0f} s), "
+                        "brute force"))
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (what lscpu prints as 'Model name')."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 # VALU issue slots per site pair in the full-energy kernel's inner loop
 # (FP64 path without the per-pair wrap, read from this build's gfx950 code
 # object: 29 full-rate VALU instructions -- displacement 3, r^2 3, the rsq
@@ -292,11 +346,10 @@ def precision_error_measure(moves: int = 3000):
     return out
 
 
-def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
-    """512 independent water boxes (64-4096 molecules, log-uniform; density
-    U[0.95, 1.05] g/cm3, T U[250, 400] K, rc 9 A) advanced together, every
-    box `sweeps` moves: whole-batch moves/s with FP32 pair terms and with
-    FP64, against the plain FP64 path on a sample of the same boxes."""
+def c2_boxes(n_boxes: int = 512, seed: int = 512):
+    """The independent water boxes advanced together: 64-4096 molecules
+    (log-uniform), density U[0.95, 1.05] g/cm3, T U[250, 400] K, rc 9 A,
+    TIP3P (seeded)."""
     import mcgpu_amd as m
     rng = np.random.default_rng(seed)
     boxes = []
@@ -305,6 +358,15 @@ def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
             m.WATER, int(n), density=float(rng.uniform(0.95, 1.05)), cutoff=9.0,
             temperature=float(rng.uniform(250, 400)), max_translate=0.15,
             max_rotate=15.0, seed=5000 + k))
+    return boxes
+
+
+def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
+    """512 independent water boxes (c2_boxes) advanced together, every box
+    `sweeps` moves: whole-batch moves/s with FP32 pair terms and with FP64,
+    against the plain FP64 path on a sample of the same boxes."""
+    import mcgpu_amd as m
+    boxes = c2_boxes(n_boxes, seed)
     out = {"boxes": n_boxes, "sweeps": sweeps,
            "molecules_total": int(sum(b.desc().n_molecules for b in boxes))}
     for name, prec in (("mixed", m.MIXED), ("fp64", m.FP64)):
@@ -333,27 +395,38 @@ def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
     return out
 
 
-def pmc_traffic():
+def pmc_traffic(steps: int, warmup: int):
     """HBM bytes per launch of the batch kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
+    counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled), used only
+    when that profile was taken on this same workload (the same --steps and
+    --warmup); None otherwise."""
     p = ROOT / "profiles" / "k_batch_serving_pmc.txt"
     if not p.exists():
         return None
-    m_ = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
-    return float(m_.group(1)) if m_ else None
+    text = p.read_text()
+    args = re.search(r"bench_args:\s*--steps (\d+) --warmup (\d+)", text)
+    val = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    if not args or not val or (int(args.group(1)), int(args.group(2))) != (steps, warmup):
+        return None
+    return float(val.group(1))
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000, help="jobs in the timed stream")
-    ap.add_argument("--warmup", type=int, default=32, help="warm-up jobs (untimed)")
+    ap.add_argument("--steps", type=int, default=20,
+                    help=f"timed steps; a step is {JOBS_PER_STEP} jobs of the stream")
+    ap.add_argument("--warmup", type=int, default=2,
+                    help=f"untimed warm-up steps ({JOBS_PER_STEP} jobs each, "
+                    "at most 2000 moves per job, all arriving at once)")
     ap.add_argument("--sweeps", type=int, default=2000, help="max moves per box per tick")
     ap.add_argument("--tick-us", type=float, default=2000.0,
                     help="per-launch time limit in microseconds")
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch512", action="store_true",
+                    help="skip the 512-water-box measurement")
     ap.add_argument("--full-energy-only", action="store_true",
                     help="measure only the full-energy kernel and print its JSON")
     ap.add_argument("--precision-only", action="store_true",
@@ -381,8 +454,10 @@ def main() -> int:
         return 0
 
     t_setup = time.perf_counter()
-    stream = make_stream(args.warmup + args.steps)
-    warm, timed = stream[:args.warmup], stream[args.warmup:]
+    # one continuous stream: the warm-up steps' jobs, then the timed steps'
+    n_warm, n_timed = args.warmup * JOBS_PER_STEP, args.steps * JOBS_PER_STEP
+    stream = make_stream(n_warm + n_timed)
+    warm, timed = stream[:n_warm], stream[n_warm:]
     pool = BoxPool()
     for j in stream:
         pool.get(j["kind"], j["n"])
@@ -392,14 +467,18 @@ def main() -> int:
 
     os.environ.setdefault("MCGPU_SERVER_LOG", "10")
     # warm-up: same server path, saturated (all warm-up jobs at t=0)
-    wj = [dict(j, arrival=0.0, steps=min(j["steps"], 2000)) for j in warm]
-    _, wst = serve(wj, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
+    wticks = 0
+    if warm:
+        wj = [dict(j, arrival=0.0, steps=min(j["steps"], 2000)) for j in warm]
+        _, wst = serve(wj, pool, t_shift=False, sweeps=args.sweeps, tick_us=args.tick_us)
+        wticks = wst.ticks
+    m.check(m.lib().mcgpu_device_synchronize(), "mcgpu_device_synchronize")
     print("# warm-up done", file=sys.stderr, flush=True)
 
-    # timed region: exactly K jobs of the stream
-    t0 = time.perf_counter()
+    # timed region: exactly K steps of the stream (K x JOBS_PER_STEP jobs),
+    # on the server's clock from its start to the last job's completion
+    # (every job's result is read back, so the device is idle at both ends)
     res, st = serve(timed, pool, sweeps=args.sweeps, tick_us=args.tick_us)
-    wall = time.perf_counter() - t0
     bad = [r for r in res if r.status != 0]
     if bad:
         print(f"bench.py: {len(bad)} jobs failed", file=sys.stderr)
@@ -423,7 +502,7 @@ def main() -> int:
     alg_bytes = 24.0 * (st.work_candidates + st.work_sites)
     achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
     per_launch = alg_bytes / max(st.ticks, 1)
-    traffic = pmc_traffic()
+    traffic = pmc_traffic(args.steps, args.warmup)
 
     plain_rate, plain_jobs, plain_moves = plain_sample(timed, pool, args.plain_seconds)
     print(f"# plain FP64 path: {plain_rate:.4g} moves/s over {plain_jobs} jobs",
@@ -434,19 +513,27 @@ def main() -> int:
     prec = precision_error_measure()
     print(f"# FP32-pair dE error: {json.dumps(prec)}", file=sys.stderr, flush=True)
 
+    b512 = None
+    if not args.no_batch512:
+        b512 = batch512_measure()
+        print(f"# 512 water boxes: {json.dumps(b512)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(timed, pool, args.cpu_seconds)
+        cpu["cpu_model"] = cpu_model()
         cpu["cores_available"] = os.cpu_count()
+        cpu["all_cores"] = cpu_baseline_all_cores(timed, pool, args.cpu_seconds)
+        print(f"# CPU baseline: {json.dumps(cpu)}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "moves/s",
         "n_gpus": 1,
-        "steps": len(timed),
-        "warmup": len(warm),
-        "ms_per_step": 1e3 * st.wall_s / len(timed),
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * st.wall_s / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -456,7 +543,8 @@ def main() -> int:
             "workload": ("serving stream: Poisson 200 jobs/s, water (c2 generator) "
                          "and 16-site bicyclic (c1 generator) boxes, 128-16384 "
                          "molecules, 1e3-1e5 steps, NVT/NPT, FP64/FP32-pair mixed"),
-            "jobs": len(timed), "max_sweeps_per_tick": args.sweeps,
+            "jobs": len(timed), "jobs_per_step": JOBS_PER_STEP,
+            "first_job": n_warm, "moves": moves, "max_sweeps_per_tick": args.sweeps,
             "tick_budget_us": args.tick_us,
             "parallelism": "boxes in one persistent launch",
         },
@@ -479,10 +567,11 @@ def main() -> int:
             "avg_launch_ms": st.kernel_ms_total / max(st.ticks, 1),
             # graph launches of the whole process (warm-up, timed and
             # capacity serving), to normalise per-process counter totals
-            "graph_launches_in_run": int(wst.ticks + st.ticks + cst.ticks),
+            "graph_launches_in_run": int(wticks + st.ticks + cst.ticks),
         },
         "full_energy": full,
         "fp32_pair_de_error_kcal_mol": prec,
+        "batch512": b512,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
