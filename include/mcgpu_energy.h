@@ -218,6 +218,44 @@ int mcgpu_box_footprint(const mcgpu_box_desc* desc,
 int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* desc,
                     const mcgpu_run_params* params, int32_t* slot);
 int mcgpu_batch_remove(mcgpu_batch* b, int32_t slot);
+
+/* Everything a box needs to continue its move sequence exactly where it
+ * stopped: its generator state, running energy and LRC, the NPT step width,
+ * and counters since its first start.  The reference re-seeds on resume
+ * (Simulation.cpp:51, with the seed its .state reader loses), so its resumed
+ * runs start a new stream; a box added with this record continues the old
+ * one, and its moves, decisions and energies equal those of the run that was
+ * not interrupted. */
+typedef struct mcgpu_resume_state {
+  int32_t rng_words[31];   /* glibc TYPE_3 state words */
+  int32_t rng_front;       /* front index (rear = front - 3 mod 31) */
+  int32_t rng_rear;
+  int32_t volume_pending;  /* NPT: the volume move due at steps_done is not
+                              made yet (a run that ended on a volume stop) */
+  int64_t steps_done;      /* moves since the first start */
+  int64_t accepted;
+  int64_t volume_moves;    /* NPT volume moves since the first start */
+  int64_t volume_accepted;
+  double energy;           /* running total energy */
+  double lrc;              /* current LRC (NPT: rescaled with 1/V) */
+  double volume_step;      /* NPT half-width of the volume step, A^3 */
+  /* the run's ensemble settings, for a caller that continues the run from
+     a .state file alone (mcgpu_batch_add_resumed takes its params) */
+  int32_t ensemble;        /* MCGPU_NVT / MCGPU_NPT */
+  int32_t volume_interval;
+  double pressure;         /* kcal/mol/A^3 */
+} mcgpu_resume_state;
+/* Reads a box's resume record (after the work queued so far). */
+int mcgpu_batch_get_resume(mcgpu_batch* b, int32_t slot, mcgpu_resume_state* out);
+/* mcgpu_batch_add for a box that continues from a resume record: params
+ * give the run (its steps are the moves of this run; params->seed and
+ * energy_offset are not used); the generator, running energy, LRC and NPT
+ * step width come from r, and the NPT volume schedule keeps its phase
+ * (volume moves after every volume_interval moves counted from the first
+ * start). */
+int mcgpu_batch_add_resumed(mcgpu_batch* b, const mcgpu_box_desc* desc,
+                            const mcgpu_run_params* params,
+                            const mcgpu_resume_state* r, int32_t* slot);
 /* Advances every active box by up to `sweeps` moves in one persistent
  * launch (captured and replayed as a HIP graph when the box set is
  * unchanged).  Asynchronous unless sync != 0. */

@@ -55,9 +55,17 @@ int mcgpu_hostbox_coords(mcgpu_hostbox* h, double** x, double** y, double** z);
 /* Sets the box edges (e.g. after NPT volume moves, before a state write). */
 int mcgpu_hostbox_set_edges(mcgpu_hostbox* h, const double* edges3);
 
-/* .state files, full precision (%.17g) so a reload is bit exact. */
+/* .state files, full precision (%.17g) so a reload is bit exact.  A box
+ * that carries a resume record (mcgpu_hostbox_set_resume) gets it written on
+ * a last line that the reference's reader skips (StateScanner.cpp:54-195);
+ * mcgpu_state_read restores it. */
 int mcgpu_state_write(mcgpu_hostbox* h, const char* path, int64_t step);
 int mcgpu_state_read(const char* path, mcgpu_hostbox** out, int64_t* step);
+/* Attaches (r != NULL) or clears the box's resume record, and reads it
+ * (*present = 0 when the box has none). */
+int mcgpu_hostbox_set_resume(mcgpu_hostbox* h, const mcgpu_resume_state* r);
+int mcgpu_hostbox_get_resume(mcgpu_hostbox* h, mcgpu_resume_state* r,
+                             int32_t* present);
 /* PDB snapshot in the reference's column layout (Simulation.cpp:362-417). */
 int mcgpu_pdb_write(mcgpu_hostbox* h, const char* path);
 

@@ -78,6 +78,17 @@ class VolumeRecord(C.Structure):
     ]
 
 
+class ResumeState(C.Structure):
+    _fields_ = [
+        ("rng_words", C.c_int32 * 31), ("rng_front", C.c_int32),
+        ("rng_rear", C.c_int32), ("volume_pending", C.c_int32),
+        ("steps_done", C.c_int64), ("accepted", C.c_int64),
+        ("volume_moves", C.c_int64), ("volume_accepted", C.c_int64),
+        ("energy", C.c_double), ("lrc", C.c_double), ("volume_step", C.c_double),
+        ("ensemble", C.c_int32), ("volume_interval", C.c_int32), ("pressure", C.c_double),
+    ]
+
+
 class WorkloadParams(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("n_molecules", C.c_int32), ("n_second", C.c_int32),

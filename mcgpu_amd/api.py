@@ -7,8 +7,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import (BoxDesc, BoxStats, ConfigInfo, JobResult, MoveResult, PlainResult,
-               RunParams, ServerConfig, ServerStats, VolumeRecord, WorkloadParams,
-               check, lib)
+               ResumeState, RunParams, ServerConfig, ServerStats, VolumeRecord,
+               WorkloadParams, check, lib)
 
 __all__ = [
     "HostBox", "DeviceBox", "Batch", "Server", "plain_run", "strategy_run",
@@ -78,6 +78,32 @@ class HostBox:
     def write_state(self, path: str, step: int = 0) -> None:
         check(lib().mcgpu_state_write(self._h, str(path).encode(),
                                       C.c_int64(step)), "mcgpu_state_write")
+
+    @property
+    def resume(self):
+        """The box's resume record (ResumeState) or None."""
+        r = ResumeState()
+        present = C.c_int32()
+        check(lib().mcgpu_hostbox_get_resume(self._h, C.byref(r), C.byref(present)),
+              "mcgpu_hostbox_get_resume")
+        return r if present.value else None
+
+    @resume.setter
+    def resume(self, r) -> None:
+        check(lib().mcgpu_hostbox_set_resume(self._h, C.byref(r) if r is not None else None),
+              "mcgpu_hostbox_set_resume")
+
+    def set_coords(self, x, y, z, edges=None) -> None:
+        """Overwrites the box's coordinates (and edges)."""
+        px, py, pz = (C.POINTER(C.c_double)(), C.POINTER(C.c_double)(),
+                      C.POINTER(C.c_double)())
+        check(lib().mcgpu_hostbox_coords(self._h, C.byref(px), C.byref(py), C.byref(pz)))
+        n = self.desc().n_sites
+        for dst, src in ((px, x), (py, y), (pz, z)):
+            np.ctypeslib.as_array(dst, shape=(n,))[:] = np.asarray(src, np.float64)
+        if edges is not None:
+            e = (C.c_double * 3)(*[float(v) for v in edges])
+            check(lib().mcgpu_hostbox_set_edges(self._h, e))
 
     def free(self) -> None:
         if self._h:
@@ -311,7 +337,9 @@ class Batch:
             precision: int = 0, ensemble: int = 0, volume_interval: int = 0,
             pressure: float = 0.0, max_volume_delta: float = 0.0,
             energy_offset: float | None = None, neighbor: int = 0,
-            record_trace: bool = False) -> int:
+            record_trace: bool = False, resume=None) -> int:
+        """Adds a box; with resume (a ResumeState) it continues that record's
+        move sequence (mcgpu_batch_add_resumed)."""
         d = box.desc()
         if energy_offset is None:
             energy_offset = lrc_for_run(d, ensemble)
@@ -319,8 +347,13 @@ class Batch:
                       volume_interval, steps, pressure, max_volume_delta,
                       energy_offset, neighbor, int(record_trace))
         slot = C.c_int32()
-        check(lib().mcgpu_batch_add(self._b, C.byref(d), C.byref(p), C.byref(slot)),
-              "mcgpu_batch_add")
+        if resume is not None:
+            check(lib().mcgpu_batch_add_resumed(self._b, C.byref(d), C.byref(p),
+                                                C.byref(resume), C.byref(slot)),
+                  "mcgpu_batch_add_resumed")
+        else:
+            check(lib().mcgpu_batch_add(self._b, C.byref(d), C.byref(p), C.byref(slot)),
+                  "mcgpu_batch_add")
         self._keep[slot.value] = (box, d.n_sites, steps)
         return slot.value
 
@@ -354,6 +387,12 @@ class Batch:
                                       C.c_int64(cap), C.byref(n)), "mcgpu_batch_trace")
         k = n.value
         return RunTrace(tm[:k], td[:k], ta[:k])
+
+    def get_resume(self, slot: int) -> ResumeState:
+        r = ResumeState()
+        check(lib().mcgpu_batch_get_resume(self._b, int(slot), C.byref(r)),
+              "mcgpu_batch_get_resume")
+        return r
 
     def volume_trace(self, slot: int) -> dict:
         """The NPT volume moves of a record_trace box, as arrays: step,

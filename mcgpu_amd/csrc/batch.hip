@@ -413,10 +413,12 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
 }
 
 // Start energy of a newly added box: pairs (reduced by k_reduce2 into
-// total[0]) plus the offset (the LRC).
-__global__ void k_set_start_energy(BoxCtl* c, const double* total, double offset) {
+// total[0]) plus the offset (the LRC), or, for a box that continues from a
+// resume record, that record's running energy.
+__global__ void k_set_start_energy(BoxCtl* c, const double* total, double offset,
+                                   int resumed, double energy) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    c->energy = total[0] + offset;
+    c->energy = resumed ? energy : total[0] + offset;
     c->start_energy = c->energy;
   }
 }
@@ -837,6 +839,8 @@ struct BatchSlot {
   int64_t tune_key = 0;    // box class for the work-group-size tuner
   int trial = -1;          // group being measured for the class (-1: none)
   int64_t steps_at_launch = 0;
+  // counters before this run (a box added from a resume record)
+  int64_t base_steps = 0, base_acc = 0, base_vol = 0, base_vacc = 0;
   bool dirty = false;      // the device's control block is newer than h_ctl
   bool vol_queued = false; // in the volume pass queued at the last boundary
 };
@@ -1288,10 +1292,17 @@ extern "C" int mcgpu_box_footprint(const mcgpu_box_desc* d,
 
 // Admission: the box's arrays are uploaded and its cells, start energy and
 // control block set up on the set-up stream, beside a tick in flight (a
-// tick only touches the boxes it runs); the box joins the next tick.
-extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
-                               const mcgpu_run_params* p, int32_t* slot_out) {
+// tick only touches the boxes it runs); the box joins the next tick.  With
+// a resume record r the box continues that record's generator stream,
+// running energy, LRC and NPT schedule.
+static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_run_params* p,
+                          const mcgpu_resume_state* r, int32_t* slot_out) {
   if (!b || !d || !p || !slot_out) return MCGPU_ERR_INVALID;
+  if (r && (r->rng_front < 0 || r->rng_front > 30 || r->rng_rear < 0 || r->rng_rear > 30 ||
+            (r->rng_front - r->rng_rear + 31) % 31 != 3 || r->steps_done < 0)) {
+    set_error("mcgpu_batch_add_resumed: bad resume record");
+    return MCGPU_ERR_INVALID;
+  }
   const std::string why = validate_desc(*d);
   if (!why.empty()) {
     set_error("mcgpu_batch_add: " + why);
@@ -1405,6 +1416,7 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   // fixed-width volume step: a symmetric proposal, so the acceptance test
   // needs no Hastings factor
   E.dv = p->max_volume_delta * (d->box[0] * d->box[1] * d->box[2]);
+  if (r && r->volume_step > 0) E.dv = r->volume_step;
   E.cutoff = d->cutoff;
   E.spread = pt.spread;
   E.vol_interval = p->volume_interval;
@@ -1414,7 +1426,17 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   E.tr_acc = (int8_t*)(base + L.tra);
   E.tr_vol = L.trv_cap > 0 ? (mcgpu_volume_record*)(base + L.trv) : nullptr;
   E.tr_vol_cap = L.trv_cap;
-  rng31_seed(b->h_rng[s], p->seed);
+  if (r) {
+    for (int k = 0; k < 31; k++) b->h_rng[s].s[k] = r->rng_words[k];
+    b->h_rng[s].f = r->rng_front;
+    b->h_rng[s].r = r->rng_rear;
+    S.base_steps = r->steps_done;
+    S.base_acc = r->accepted;
+    S.base_vol = r->volume_moves;
+    S.base_vacc = r->volume_accepted;
+  } else {
+    rng31_seed(b->h_rng[s], p->seed);
+  }
   const hipStream_t st = b->sstream;
   const int32_t zero[3] = {0, 0, 0};
   struct Copy {
@@ -1471,20 +1493,71 @@ extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
   BoxCtl& c = b->h_ctl[s];
   c = BoxCtl{};
   for (int k = 0; k < 3; k++) c.L[k] = d->box[k];
-  c.lrc = p->energy_offset;
+  c.lrc = r ? r->lrc : p->energy_offset;
   c.steps_target = p->steps;
-  c.next_stop = (p->ensemble == MCGPU_NPT) ? std::min<int64_t>(p->steps, p->volume_interval)
-                                           : p->steps;
+  c.next_stop = p->steps;
+  if (p->ensemble == MCGPU_NPT) {
+    // volume moves after every volume_interval moves since the first start;
+    // one due where a previous run stopped comes first
+    const int64_t prior = r ? r->steps_done : 0;
+    const int64_t due = (r && r->volume_pending) ? 0 : p->volume_interval - prior % p->volume_interval;
+    c.next_stop = std::min<int64_t>(p->steps, due);
+  }
   c.status = p->steps > 0 ? kRunning : kDone;
   if ((rc = upload_ctl(b, s, st))) {
     free_slot(b, s);
     return rc;
   }
   hipLaunchKernelGGL(k_set_start_energy, dim3(1), dim3(64), 0, st, b->d_ctl + s,
-                     (const double*)E.total, p->energy_offset);
+                     (const double*)E.total, p->energy_offset, r ? 1 : 0,
+                     r ? r->energy : 0.0);
   MCGPU_HIP_TRY(hipGetLastError());
   S.dirty = true;  // the device now holds the start energy
   *slot_out = s;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* d,
+                               const mcgpu_run_params* p, int32_t* slot_out) {
+  return batch_add_impl(b, d, p, nullptr, slot_out);
+}
+
+extern "C" int mcgpu_batch_add_resumed(mcgpu_batch* b, const mcgpu_box_desc* d,
+                                       const mcgpu_run_params* p, const mcgpu_resume_state* r,
+                                       int32_t* slot_out) {
+  if (!r) return MCGPU_ERR_INVALID;
+  return batch_add_impl(b, d, p, r, slot_out);
+}
+
+extern "C" int mcgpu_batch_get_resume(mcgpu_batch* b, int32_t s, mcgpu_resume_state* out) {
+  if (!b || !out || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
+    set_error("mcgpu_batch_get_resume: bad slot");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_quiesce(b);
+  if (rc) return rc;
+  const BatchSlot& S = b->slots[s];
+  const BoxCtl& c = b->h_ctl[s];
+  const Rng31& g = b->h_rng[s];
+  *out = mcgpu_resume_state{};
+  for (int k = 0; k < 31; k++) out->rng_words[k] = g.s[k];
+  out->rng_front = g.f;
+  out->rng_rear = g.r;
+  out->steps_done = S.base_steps + c.steps_done;
+  out->accepted = S.base_acc + c.accepted;
+  out->volume_moves = S.base_vol + c.vol_moves;
+  out->volume_accepted = S.base_vacc + c.vol_acc;
+  out->energy = c.energy;
+  out->lrc = c.lrc;
+  out->ensemble = S.params.ensemble;
+  out->volume_interval = S.params.volume_interval;
+  out->pressure = S.params.pressure;
+  const int vi = S.params.volume_interval;
+  if (S.params.ensemble == MCGPU_NPT && vi > 0) {
+    out->volume_step = b->h_table[s].dv;
+    out->volume_pending = out->steps_done > 0 && out->steps_done % vi == 0 &&
+                          out->volume_moves < out->steps_done / vi;
+  }
   return MCGPU_OK;
 }
 

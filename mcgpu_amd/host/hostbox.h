@@ -25,6 +25,11 @@ struct HostBox {
   std::vector<int32_t> mol_start, mol_len, mol_pidx_start, mol_pidx_count;
   std::vector<int32_t> mol_type, pidx;
 
+  // where the box's move sequence stopped (a .state written by this
+  // library after a run); absent for fresh boxes
+  bool has_resume = false;
+  mcgpu_resume_state resume{};
+
   int32_t n_mol() const { return (int32_t)mol_start.size(); }
   int32_t n_site() const { return (int32_t)x.size(); }
 

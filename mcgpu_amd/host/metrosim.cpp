@@ -287,6 +287,15 @@ int main(int argc, char** argv) {
     return EXIT_FAILURE;
   }
   if (a.steps > 0) steps = a.steps;
+  // a .state written by this program after a run carries the run's ensemble
+  mcgpu_resume_state resume{};
+  int32_t resumed = 0;
+  mcgpu_hostbox_get_resume(hb, &resume, &resumed);
+  if (resumed && resume.ensemble == MCGPU_NPT && ensemble.empty()) {
+    ensemble = "npt";
+    volume_interval = resume.volume_interval;
+    pressure_atm = resume.pressure / kAtmToKcalMolA3;
+  }
   int neighbor = MCGPU_NEIGHBOR_BRUTE;
   const std::string strat = a.strategy.empty() ? cfg_strategy : a.strategy;
   if (!strategy_neighbor(strat, neighbor)) {
@@ -336,7 +345,16 @@ int main(int argc, char** argv) {
   p.record_trace = 0;
   int32_t slot = -1;
   const auto t0 = std::chrono::steady_clock::now();
-  if (mcgpu_batch_add(batch, &desc, &p, &slot) != MCGPU_OK) die("mcgpu_batch_add");
+  // a .state written after a run of this program carries the generator
+  // state, running energy and NPT schedule: the run continues that move
+  // sequence (the reference re-seeds instead, Simulation.cpp:51)
+  if (resumed) {
+    if (mcgpu_batch_add_resumed(batch, &desc, &p, &resume, &slot) != MCGPU_OK)
+      die("mcgpu_batch_add_resumed");
+    if (a.verbose) std::printf("Continuing the move sequence of step %ld\n", start_step);
+  } else if (mcgpu_batch_add(batch, &desc, &p, &slot) != MCGPU_OK) {
+    die("mcgpu_batch_add");
+  }
 
   const std::string base = a.name.empty() ? "untitled" : a.name;
   auto save_state = [&](long step) {
@@ -346,6 +364,9 @@ int main(int argc, char** argv) {
     mcgpu_box_stats st{};
     mcgpu_batch_stats(batch, slot, &st);
     mcgpu_hostbox_set_edges(hb, st.box);
+    mcgpu_resume_state rs{};
+    if (mcgpu_batch_get_resume(batch, slot, &rs) != MCGPU_OK) die("mcgpu_batch_get_resume");
+    mcgpu_hostbox_set_resume(hb, &rs);
     std::string path = state_dir.empty() ? "" : state_dir + "/";
     path += base + "_" + std::to_string(step) + ".state";
     if (a.verbose) std::printf("Saving state file %s\n", path.c_str());

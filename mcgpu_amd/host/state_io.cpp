@@ -21,6 +21,9 @@ namespace mcgpu {
 HostBox& hostbox_ref(mcgpu_hostbox* h);
 mcgpu_hostbox* hostbox_new(HostBox&& b);
 
+// First token of the resume-record line of a .state file.
+constexpr const char* kResumeTag = "#mcgpu-resume";
+
 // Parses a primary definition line such as "1", "2,1", "[1,2],3"
 // (StateScanner.cpp:437-491, 1-based) into per-type 0-based index lists.
 std::vector<std::vector<int>> parse_primary_line(const std::string& s) {
@@ -53,6 +56,7 @@ std::vector<std::vector<int>> parse_primary_line(const std::string& s) {
 }  // namespace mcgpu
 
 using mcgpu::HostBox;
+using mcgpu::kResumeTag;
 
 extern "C" int mcgpu_state_write(mcgpu_hostbox* h, const char* path,
                                  int64_t step) {
@@ -81,7 +85,26 @@ extern "C" int mcgpu_state_write(mcgpu_hostbox* h, const char* path,
     }
     std::fprintf(f, "= Bonds\n= Dihedrals\n= Hops\n= Angles\n==\n");
   }
-  std::fclose(f);
+  if (b.has_resume) {
+    // Resume record after the last molecule, on one line that opens with
+    // '#': the reference's reader (StateScanner.cpp:54-195) takes a line
+    // that does not open with "= " or "==" in its molecule-id state as an
+    // id and keeps no molecule from it, so it loads such a file as it loads
+    // the file without this line.
+    const mcgpu_resume_state& r = b.resume;
+    std::fprintf(f, "%s %lld %lld %lld %lld %d %.17g %.17g %.17g %d %d %.17g %d %d", kResumeTag,
+                 (long long)r.steps_done, (long long)r.accepted, (long long)r.volume_moves,
+                 (long long)r.volume_accepted, r.volume_pending, r.energy, r.lrc,
+                 r.volume_step, r.ensemble, r.volume_interval, r.pressure, r.rng_front,
+                 r.rng_rear);
+    for (int k = 0; k < 31; k++) std::fprintf(f, " %d", r.rng_words[k]);
+    std::fprintf(f, "\n");
+  }
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) {
+    mcgpu::set_error(std::string("mcgpu_state_write: write failed on ") + path);
+    return MCGPU_ERR_INVALID;
+  }
   return MCGPU_OK;
 }
 
@@ -121,8 +144,28 @@ extern "C" int mcgpu_state_read(const char* path, mcgpu_hostbox** out,
   while (std::getline(in, line)) {
     const std::string hold = line.substr(0, 2);
     if (section == 0) {  // molecule id
-      if (hold == "= ") section = 1;
-      else if (line.find_first_not_of(" \t\r") == std::string::npos) break;
+      if (hold == "= ") {
+        section = 1;
+      } else if (line.compare(0, std::strlen(kResumeTag), kResumeTag) == 0) {
+        mcgpu_resume_state& r = b.resume;
+        std::istringstream ss(line.substr(std::strlen(kResumeTag)));
+        long long sd, ac, vm, va;
+        ss >> sd >> ac >> vm >> va >> r.volume_pending >> r.energy >> r.lrc >>
+            r.volume_step >> r.ensemble >> r.volume_interval >> r.pressure >> r.rng_front >>
+            r.rng_rear;
+        for (int k = 0; k < 31; k++) ss >> r.rng_words[k];
+        if (!ss) {
+          mcgpu::set_error("mcgpu_state_read: bad resume record");
+          return MCGPU_ERR_INVALID;
+        }
+        r.steps_done = sd;
+        r.accepted = ac;
+        r.volume_moves = vm;
+        r.volume_accepted = va;
+        b.has_resume = true;
+      } else if (line.find_first_not_of(" \t\r") == std::string::npos) {
+        break;
+      }
       continue;
     }
     if (section == 1) {  // type
@@ -186,6 +229,23 @@ extern "C" int mcgpu_state_read(const char* path, mcgpu_hostbox** out,
   mcgpu_hostbox* h = mcgpu::hostbox_new(std::move(b));
   if (!h) return MCGPU_ERR_NOMEM;
   *out = h;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_hostbox_set_resume(mcgpu_hostbox* h, const mcgpu_resume_state* r) {
+  if (!h) return MCGPU_ERR_INVALID;
+  HostBox& b = mcgpu::hostbox_ref(h);
+  b.has_resume = r != nullptr;
+  b.resume = r ? *r : mcgpu_resume_state{};
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_hostbox_get_resume(mcgpu_hostbox* h, mcgpu_resume_state* r,
+                                        int32_t* present) {
+  if (!h || !r || !present) return MCGPU_ERR_INVALID;
+  const HostBox& b = mcgpu::hostbox_ref(h);
+  *present = b.has_resume ? 1 : 0;
+  *r = b.resume;
   return MCGPU_OK;
 }
 

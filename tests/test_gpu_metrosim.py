@@ -72,6 +72,33 @@ def test_metrosim_state_interval_and_resume(tmp_path):
     assert "starting from step 3000" in r.stdout
     assert "Starting-Step = 3000" in (tmp_path / "more.results").read_text()
     assert not list(tmp_path.glob("more_*.state"))
+    # the .state carries the generator state and running energy: the resumed
+    # run continues the move sequence of a run that was never interrupted
+    first = metrosim([str(cfg), "-p", "-n", "3000", "--name", "a"], tmp_path)
+    straight = metrosim([str(cfg), "-p", "-n", "3500", "--name", "b"], tmp_path)
+    acc = lambda out: int(re.search(r"Accepted Moves: (\d+)", out).group(1))
+    fin = lambda out: re.search(r"Final Energy: (\S+)", out).group(1)
+    assert fin(r.stdout) == fin(straight.stdout)
+    assert acc(first.stdout) + acc(r.stdout) == acc(straight.stdout)
+
+
+def test_metrosim_npt_resume_continues_the_run(tmp_path):
+    """NPT: a run of 3000 moves stops on a volume stop (volume-interval
+    500); resumed for 500 more from its .state, it makes the pending volume
+    move first and ends where a straight 3500-move run ends."""
+    case = CASES["t3pdimone_1"]
+    cfg = write_config(tmp_path / "npt.config", *case,
+                       extra="ensemble=npt\npressure=1.0\nvolume-interval=500\n")
+    first = metrosim([str(cfg), "-n", "3000", "--name", "a"], tmp_path)
+    more = metrosim([str(tmp_path / "a_3000.state"), "-n", "500", "--name", "c", "-I", "-1"],
+                    tmp_path)
+    straight = metrosim([str(cfg), "-n", "3500", "--name", "b"], tmp_path)
+    vol = lambda out: tuple(int(v) for v in
+                            re.search(r"Volume Moves: (\d+) accepted of (\d+)", out).groups())
+    fin = lambda out: re.search(r"Final Energy: (\S+)", out).group(1)
+    assert fin(more.stdout) == fin(straight.stdout)
+    (a1, n1), (a2, n2), (a3, n3) = vol(first.stdout), vol(more.stdout), vol(straight.stdout)
+    assert (n1, n2, n3) == (5, 1, 6) and a1 + a2 == a3
 
 
 def test_metrosim_npt_run(tmp_path):
