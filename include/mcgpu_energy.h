@@ -109,6 +109,13 @@ int mcgpu_box_set_precision(mcgpu_box* box, int precision);
 /* Selects the neighbour structure and (re)builds it.  kind: MCGPU_NEIGHBOR_*.
  * skin is used for MCGPU_NEIGHBOR_VERLET only (Angstrom). */
 int mcgpu_neighbor_rebuild(mcgpu_box* box, int kind, double skin);
+/* Neighbour-structure counters (VerletCalcs::updateVerlet,
+ * VerletStep.cpp:186-207, rebuilds when a primary moves more than skin/2;
+ * this box rebuilds before the next evaluation after such a commit):
+ * builds = Verlet list builds since creation (the first included);
+ * fallbacks = evaluations that scanned every molecule although cell or
+ * Verlet lists were selected (results stay exact, only slower). */
+int mcgpu_box_neighbor_stats(mcgpu_box* box, int64_t* builds, int64_t* fallbacks);
 
 /* Whole-box coordinate transfer (GPUCopy::copyIn / copyOut).  Coordinates
  * handed in here, to mcgpu_set_molecule_coords or to mcgpu_trial_move need
@@ -202,6 +209,10 @@ typedef struct mcgpu_box_stats {
   int64_t work_sites;
   int32_t done;
   int32_t fault;  /* nonzero: box stopped (see mcgpu_last_error on add) */
+  /* Moves of a cell-list box evaluated by scanning every molecule (the
+   * lists were incomplete after a cell overflowed, or the move's cell range
+   * exceeded the scan's limit); results stay exact, only slower. */
+  int64_t neighbor_fallbacks;
 } mcgpu_box_stats;
 
 typedef struct mcgpu_batch mcgpu_batch;

@@ -123,6 +123,8 @@ typedef struct mcgpu_plain_result {
   int64_t rejected;
   double loop_seconds;    /* wall time of the step loop */
   double startup_seconds; /* wall time of the start-up energy */
+  int64_t neighbor_builds;    /* Verlet list builds (mcgpu_box_neighbor_stats) */
+  int64_t neighbor_fallbacks; /* evaluations that scanned every molecule */
 } mcgpu_plain_result;
 
 /* Runs `steps` NVT moves of box h (mutated to the final state) with the

@@ -67,6 +67,7 @@ class BoxStats(C.Structure):
         ("energy", C.c_double), ("start_energy", C.c_double),
         ("box", C.c_double * 3), ("work_candidates", C.c_int64),
         ("work_sites", C.c_int64), ("done", C.c_int32), ("fault", C.c_int32),
+        ("neighbor_fallbacks", C.c_int64),
     ]
 
 
@@ -104,7 +105,8 @@ class PlainResult(C.Structure):
         ("start_energy", C.c_double), ("lrc", C.c_double),
         ("final_energy", C.c_double), ("accepted", C.c_int64),
         ("rejected", C.c_int64), ("loop_seconds", C.c_double),
-        ("startup_seconds", C.c_double),
+        ("startup_seconds", C.c_double), ("neighbor_builds", C.c_int64),
+        ("neighbor_fallbacks", C.c_int64),
     ]
 
 

@@ -266,6 +266,14 @@ class DeviceBox:
     def enable_timing(self, on: bool = True) -> None:
         check(lib().mcgpu_box_enable_timing(self._b, int(on)))
 
+    def neighbor_stats(self):
+        """(Verlet list builds, evaluations that fell back to scanning every
+        molecule) since the box was created."""
+        b, f = C.c_int64(), C.c_int64()
+        check(lib().mcgpu_box_neighbor_stats(self._b, C.byref(b), C.byref(f)),
+              "mcgpu_box_neighbor_stats")
+        return b.value, f.value
+
     def last_kernel_ms(self) -> float:
         v = C.c_double()
         check(lib().mcgpu_last_kernel_ms(self._b, C.byref(v)))

@@ -45,6 +45,7 @@ struct BoxCtl {
   int64_t work_self;   // sites of the moving molecule (old + proposed)
   int64_t vol_moves;   // NPT volume moves tried / accepted
   int64_t vol_acc;
+  int64_t fallbacks;   // moves of a cell-list box that scanned every molecule
   double energy;
   double start_energy;
   double L[3];         // current edges
@@ -314,6 +315,7 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
     S.c.work_cand += (unsigned)(wc & 0xffffffffu);
     S.c.work_sites += (unsigned)(wc >> 32);
     S.c.work_self += 2 * len;
+    if (bv.mode == MCGPU_NB_CELL && sh.src != MCGPU_NB_CELL) S.c.fallbacks++;
     if (S.tb.record) {
       S.tb.tr_mol[step] = mol;
       S.tb.tr_de[step] = de;
@@ -1702,6 +1704,7 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   out->start_energy = c.start_energy;
   out->work_candidates = c.work_cand;
   out->work_sites = c.work_sites + c.work_self;
+  out->neighbor_fallbacks = c.fallbacks;
   for (int k = 0; k < 3; k++) out->box[k] = c.L[k];
   out->done = c.status == kDone;
   out->fault = c.fault ? c.fault : b->slots[s].fault;

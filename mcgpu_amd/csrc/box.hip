@@ -60,15 +60,17 @@ __global__ __launch_bounds__(NT) void k_move(BoxView bv, MoveArgs a,
     out->e_new = en;
     out->de = de;
     out->accepted = acc;
-    out->source = bv.mode;
+    out->source = sh.src;
     sh.flag = acc;
   }
   __syncthreads();
   if (a.metropolis && sh.flag) commit_molecule<NT>(bv, a.mol, sh);
+  // thread 0 made the commit (and its Verlet displacement test)
+  if (t == 0) out->stale = bv.mode == MCGPU_NB_VERLET ? bv.verlet.stale[0] : 0;
 }
 
 // Overwrite one molecule (changeMolecule / rollback from the host).
-__global__ __launch_bounds__(64) void k_set_mol(BoxView bv, MoveArgs a) {
+__global__ __launch_bounds__(64) void k_set_mol(BoxView bv, MoveArgs a, MoveOut* out) {
   __shared__ EvalShared<64> sh;
   const int t = threadIdx.x;
   load_molecule<64>(bv, a.mol, sh);
@@ -80,6 +82,7 @@ __global__ __launch_bounds__(64) void k_set_mol(BoxView bv, MoveArgs a) {
   }
   __syncthreads();
   commit_molecule<64>(bv, a.mol, sh);
+  if (t == 0) out->stale = bv.mode == MCGPU_NB_VERLET ? bv.verlet.stale[0] : 0;
 }
 
 __global__ void k_refresh_primary(BoxView bv, int n_pidx) {
@@ -142,6 +145,78 @@ __global__ __launch_bounds__(NT) void k_verlet_build(BoxView bv, double rv2) {
     if (threadIdx.x == 0) {
       v.count[i] = n;
       if (n > v.cap) atomicOr(v.stale, 2);  // capacity exceeded
+    }
+    __syncthreads();
+  }
+}
+
+// Verlet list of molecule i built from a cell grid whose cells are at least
+// rv wide (>= 3 per edge): the candidates are the molecules binned in the 27
+// cells around i's first primary.  Hits are gathered in LDS and written in
+// increasing j, the order the all-molecule build gives, so the list (and the
+// summation order of every energy taken from it) does not depend on the
+// grid.  More than kVerletGather hits for one molecule sets stale bit 4 (the
+// host then uses the all-molecule build).
+constexpr int kVerletGather = 2048;
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_verlet_build_cells(BoxView bv, double rv2) {
+  __shared__ int32_t scan[NT / 64];
+  __shared__ int32_t hits[kVerletGather];
+  const VerletList& v = bv.verlet;
+  const CellGrid& g = bv.cells;
+  for (int i = blockIdx.x; i < bv.n_mol; i += gridDim.x) {
+    const int pi = bv.mol_pidx_start[i];
+    const double xi = bv.px[pi], yi = bv.py[pi], zi = bv.pz[pi];
+    if (threadIdx.x == 0) {
+      v.ref[3 * i] = xi;
+      v.ref[3 * i + 1] = yi;
+      v.ref[3 * i + 2] = zi;
+    }
+    const int cx = cell_coord(xi, g.inv_w[0], g.n[0]);
+    const int cy = cell_coord(yi, g.inv_w[1], g.n[1]);
+    const int cz = cell_coord(zi, g.inv_w[2], g.n[2]);
+    int n = 0;
+    for (int q = 0; q < 27; q++) {
+      int ax = cx + q % 3 - 1, ay = cy + (q / 3) % 3 - 1, az = cz + q / 9 - 1;
+      ax += ax < 0 ? g.n[0] : (ax >= g.n[0] ? -g.n[0] : 0);
+      ay += ay < 0 ? g.n[1] : (ay >= g.n[1] ? -g.n[1] : 0);
+      az += az < 0 ? g.n[2] : (az >= g.n[2] ? -g.n[2] : 0);
+      const int c = cell_index(g, ax, ay, az);
+      const int cnt = min(g.count[c], g.cap);
+      for (int base = 0; base < cnt; base += NT) {
+        const int k = base + (int)threadIdx.x;
+        bool f = false;
+        int j = -1;
+        if (k < cnt) {
+          j = g.slots[(size_t)c * g.cap + k];
+          if (j != i) {
+            const int pj = bv.mol_pidx_start[j];
+            const double dx = wrap1(bv.px[pj] - xi, bv.L[0]);
+            const double dy = wrap1(bv.py[pj] - yi, bv.L[1]);
+            const double dz = wrap1(bv.pz[pj] - zi, bv.L[2]);
+            f = dx * dx + dy * dy + dz * dz <= rv2;
+          }
+        }
+        int total;
+        const int pos = block_scan_flag<NT>(f, scan, total);
+        if (f && n + pos < kVerletGather) hits[n + pos] = j;
+        n += total;
+      }
+    }
+    __syncthreads();
+    const int m = min(n, kVerletGather);
+    // rank of each hit among the gathered ones (ids are distinct)
+    for (int k = threadIdx.x; k < m; k += NT) {
+      const int j = hits[k];
+      int rank = 0;
+      for (int q = 0; q < m; q++) rank += hits[q] < j ? 1 : 0;
+      if (rank < v.cap) v.list[(size_t)i * v.cap + rank] = j;
+    }
+    if (threadIdx.x == 0) {
+      v.count[i] = n;
+      if (n > v.cap) atomicOr(v.stale, 2);  // capacity exceeded
+      if (n > kVerletGather) atomicOr(v.stale, 4);
     }
     __syncthreads();
   }
@@ -357,6 +432,12 @@ struct mcgpu_box {
   std::vector<int32_t> h_mol_start, h_mol_len;
   std::vector<int16_t> h_pf;  // first primary's offset within each molecule
   double reach = 0.0;         // see ParamTable::reach (only ever grows)
+  // Verlet mode: a commit moved a primary past skin/2 (rebuild before the
+  // next evaluation), or a set-molecule call may have (read its flag first)
+  bool verlet_dirty = false, verlet_check = false;
+  int64_t verlet_builds = 0;  // list builds since the box was created
+  int64_t fallbacks = 0;      // evaluations that scanned every molecule
+                              // although cell or Verlet lists were selected
 };
 
 namespace {
@@ -404,9 +485,11 @@ int refresh_primary(mcgpu_box* b) {
   return MCGPU_OK;
 }
 
-int build_cells(mcgpu_box* b) {
+// Bins every molecule by its first primary into cells at least
+// `reach + 2 spread` wide (cell_geometry), slots ordered by molecule id.
+int bin_cells(mcgpu_box* b, double reach) {
   CellGrid& g = b->bv.cells;
-  cell_geometry(b->bv.L, std::sqrt(b->bv.cutoff2), b->spread, g.n, g.inv_w);
+  cell_geometry(b->bv.L, reach, b->spread, g.n, g.inv_w);
   const int ncell = g.n[0] * g.n[1] * g.n[2];
   if (ncell > b->ncell_alloc) {
     if (b->cell_count) MCGPU_HIP_TRY(hipFree(b->cell_count));
@@ -451,21 +534,41 @@ int build_cells(mcgpu_box* b) {
   return MCGPU_ERR_STATE;
 }
 
+int build_cells(mcgpu_box* b) { return bin_cells(b, std::sqrt(b->bv.cutoff2)); }
+
+// Verlet lists (VerletCalcs::createVerlet/fillVerlet, VerletStep.cpp:119-184,
+// done correctly): every j whose first primary lies within rv = rc +
+// 2 spread + skin of i's.  Candidates come from cells of width >= rv when
+// the box holds at least 3 per edge (O(N) work), else from every molecule.
 int build_verlet(mcgpu_box* b) {
   VerletList& v = b->bv.verlet;
   const double rv = std::sqrt(b->bv.cutoff2) + 2.0 * b->spread + b->skin;
   v.skin_half2 = 0.25 * b->skin * b->skin;
+  int n[3];
+  double iw[3];
+  cell_geometry(b->bv.L, std::sqrt(b->bv.cutoff2) + b->skin, b->spread, n, iw);
+  bool from_cells = n[0] >= 3 && n[1] >= 3 && n[2] >= 3;
+  if (from_cells) {
+    const int rc = bin_cells(b, std::sqrt(b->bv.cutoff2) + b->skin);
+    if (rc) return rc;
+  }
+  b->verlet_builds++;
   for (int attempt = 0; attempt < 6; attempt++) {
     MCGPU_HIP_TRY(hipMemsetAsync(v.stale, 0, sizeof(int32_t), b->stream));
     const int grid = std::min(b->bv.n_mol, 4096);
-    hipLaunchKernelGGL(k_verlet_build<256>, dim3(grid), dim3(256), 0, b->stream,
-                       b->bv, rv * rv);
+    if (from_cells)
+      hipLaunchKernelGGL(k_verlet_build_cells<256>, dim3(grid), dim3(256), 0, b->stream,
+                         b->bv, rv * rv);
+    else
+      hipLaunchKernelGGL(k_verlet_build<256>, dim3(grid), dim3(256), 0, b->stream,
+                         b->bv, rv * rv);
     MCGPU_HIP_TRY(hipGetLastError());
     int32_t st = 0;
     MCGPU_HIP_TRY(hipMemcpyAsync(&st, v.stale, sizeof(int32_t),
                                  hipMemcpyDeviceToHost, b->stream));
     MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
     if (!st) return MCGPU_OK;
+    if (st & 4) from_cells = false;  // a molecule gathered more than LDS holds
     std::vector<int32_t> cnt(b->bv.n_mol);
     MCGPU_HIP_TRY(hipMemcpy(cnt.data(), v.count, sizeof(int32_t) * cnt.size(),
                             hipMemcpyDeviceToHost));
@@ -495,6 +598,20 @@ int rebuild_neighbors(mcgpu_box* b) {
 int launch_move(mcgpu_box* b, const MoveArgs& a) {
   const bool big = (b->bv.mode == MCGPU_NB_BRUTE) && b->bv.n_mol > 2048;
   const bool mixed = b->precision == MCGPU_MIXED;
+  if (b->bv.mode == MCGPU_NB_VERLET) {
+    // VerletCalcs::updateVerlet (VerletStep.cpp:186-207): once a primary has
+    // moved more than skin/2 since the build, rebuild before evaluating
+    if (b->verlet_check) {
+      MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+      if (b->h_out->stale) b->verlet_dirty = true;
+      b->verlet_check = false;
+    }
+    if (b->verlet_dirty) {
+      const int rc = build_verlet(b);
+      if (rc) return rc;
+      b->verlet_dirty = false;
+    }
+  }
   if (b->ev0) MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   if (big) {
     if (mixed)
@@ -511,6 +628,8 @@ int launch_move(mcgpu_box* b, const MoveArgs& a) {
   if (b->ev1) MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   if (b->ev0) MCGPU_HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  if (b->bv.mode != MCGPU_NB_BRUTE && b->h_out->source == MCGPU_NB_BRUTE) b->fallbacks++;
+  if (b->bv.mode == MCGPU_NB_VERLET && b->h_out->stale) b->verlet_dirty = true;
   return MCGPU_OK;
 }
 
@@ -793,8 +912,17 @@ extern "C" int mcgpu_set_molecule_coords(mcgpu_box* b, int32_t mol,
   MoveArgs a{};
   a.mol = mol;
   if ((rc = fill_coords(b, mol, xyz, a)) != MCGPU_OK) return rc;
-  hipLaunchKernelGGL(k_set_mol, dim3(1), dim3(64), 0, b->stream, b->bv, a);
+  hipLaunchKernelGGL(k_set_mol, dim3(1), dim3(64), 0, b->stream, b->bv, a, b->d_out);
   MCGPU_HIP_TRY(hipGetLastError());
+  // the commit's Verlet displacement test is read before the next evaluation
+  if (b->bv.mode == MCGPU_NB_VERLET) b->verlet_check = true;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_box_neighbor_stats(mcgpu_box* b, int64_t* builds, int64_t* fallbacks) {
+  if (!b || !builds || !fallbacks) return MCGPU_ERR_INVALID;
+  *builds = b->verlet_builds;
+  *fallbacks = b->fallbacks;
   return MCGPU_OK;
 }
 

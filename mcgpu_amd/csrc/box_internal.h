@@ -32,6 +32,8 @@ struct MoveOut {
   double de;
   int32_t accepted;
   int32_t source;  // neighbour source used (MCGPU_NB_*)
+  int32_t stale;   // Verlet mode: a committed primary left its skin
+  int32_t reserved;
 };
 
 // Molecule signatures -> site parameter table, plus per-molecule base index.

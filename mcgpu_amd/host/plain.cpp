@@ -260,7 +260,7 @@ extern "C" int mcgpu_plain_run(mcgpu_hostbox* h, int32_t precision,
   out->final_energy = energy;
   out->accepted = acc;
   out->rejected = steps - acc;
-  return MCGPU_OK;
+  return mcgpu_box_neighbor_stats(dev, &out->neighbor_builds, &out->neighbor_fallbacks);
 }
 
 extern "C" int mcgpu_strategy_run(mcgpu_hostbox* h, int64_t steps,
