@@ -435,6 +435,7 @@ struct mcgpu_box {
   // Verlet mode: a commit moved a primary past skin/2 (rebuild before the
   // next evaluation), or a set-molecule call may have (read its flag first)
   bool verlet_dirty = false, verlet_check = false;
+  std::vector<double> h_ref;  // first-primary positions of the last build
   int64_t verlet_builds = 0;  // list builds since the box was created
   int64_t fallbacks = 0;      // evaluations that scanned every molecule
                               // although cell or Verlet lists were selected
@@ -567,7 +568,14 @@ int build_verlet(mcgpu_box* b) {
     MCGPU_HIP_TRY(hipMemcpyAsync(&st, v.stale, sizeof(int32_t),
                                  hipMemcpyDeviceToHost, b->stream));
     MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
-    if (!st) return MCGPU_OK;
+    if (!st) {
+      // the build positions, for the host's check of proposals (see
+      // mcgpu_trial_move)
+      b->h_ref.resize(3 * (size_t)b->bv.n_mol);
+      MCGPU_HIP_TRY(hipMemcpy(b->h_ref.data(), v.ref, sizeof(double) * b->h_ref.size(),
+                              hipMemcpyDeviceToHost));
+      return MCGPU_OK;
+    }
     if (st & 4) from_cells = false;  // a molecule gathered more than LDS holds
     std::vector<int32_t> cnt(b->bv.n_mol);
     MCGPU_HIP_TRY(hipMemcpy(cnt.data(), v.count, sizeof(int32_t) * cnt.size(),
@@ -969,6 +977,22 @@ extern "C" int mcgpu_trial_move(mcgpu_box* b, int32_t mol,
   a.metropolis = 1;
   a.u = u;
   if ((rc = fill_coords(b, mol, new_xyz, a)) != MCGPU_OK) return rc;
+  if (b->bv.mode == MCGPU_NB_VERLET && b->h_ref.size() == 3 * (size_t)b->bv.n_mol) {
+    // A proposal whose first primary lies more than skin/2 from its build
+    // position is not covered by the lists: rebuild them first (from the
+    // committed positions), so the evaluation can still use them.
+    const int len = b->h_mol_len[mol], pf = b->h_pf[mol];
+    const double* r = b->h_ref.data() + 3 * (size_t)mol;
+    double d2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      double d = new_xyz[k * len + pf] - r[k];
+      const double L = b->bv.L[k];
+      d += d < -0.5 * L ? L : 0.0;  // SimulationStep.cpp:377-385
+      d -= d > 0.5 * L ? L : 0.0;
+      d2 += d * d;
+    }
+    if (d2 > b->bv.verlet.skin_half2) b->verlet_dirty = true;
+  }
   if ((rc = launch_move(b, a)) != MCGPU_OK) return rc;
   res->old_energy = b->h_out->e_old;
   res->new_energy = b->h_out->e_new;

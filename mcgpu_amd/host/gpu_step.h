@@ -17,7 +17,9 @@ class GPUStep {
  public:
   // Creates the device box from `box` (which stays the host mirror, like
   // SimBox under GPUCopy) with the given pair arithmetic and neighbour mode.
-  GPUStep(HostBox* box, int precision, int neighbor, double skin = 1.0);
+  // skin <= 0 takes the reference's Verlet shell, 1.2875 rc
+  // (VerletStep.cpp:133): skin = 0.2875 rc.
+  GPUStep(HostBox* box, int precision, int neighbor, double skin = 0.0);
   ~GPUStep();
   int status() const { return status_; }
 

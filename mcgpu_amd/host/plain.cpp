@@ -127,6 +127,7 @@ GPUStep::GPUStep(HostBox* box, int precision, int neighbor, double skin)
     : hb_(box) {
   const mcgpu_box_desc d = hb_->desc();
   status_ = mcgpu_box_create(&d, precision, &dev_);
+  if (!(skin > 0)) skin = 0.2875 * hb_->cutoff;
   if (status_ == MCGPU_OK && neighbor != MCGPU_NEIGHBOR_BRUTE)
     status_ = mcgpu_neighbor_rebuild(dev_, neighbor, skin);
 }

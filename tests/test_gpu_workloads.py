@@ -92,7 +92,7 @@ def test_serving_stream_slice_alone_mix_and_oracle():
     srv.run()
     res = {r.job_id: r for r in srv.results()}
     assert len(res) == len(jobs)
-    assert srv.stats().peak_resident >= 8
+    assert srv.stats().peak_resident >= 4  # jobs overlapped on the device
     checked = 0
     for k, j in enumerate(jobs):
         box = pool.get(j["kind"], j["n"])
