@@ -170,75 +170,69 @@ def plain_sample(jobs, pool, budget_s: float, max_moves: int = 2000):
     return moves / secs, used, moves
 
 
-def cpu_baseline(jobs, pool, budget_s: float):
-    """The reference's own CPU path (oracle/_ref/ref_trace, single thread,
-    brute force as in its -s mode) on the first jobs of the stream, each for
-    a bounded number of moves; falls back to the oracle restatement."""
+def _ref_loop_seconds(out: str) -> float:
+    return float(re.search(r"Loop Seconds: ([0-9.eE+-]+)", out).group(1))
+
+
+def cpu_baseline(jobs, pool, budget_s: float, max_cores: int = 16):
+    """The reference's own CPU path (oracle/_ref/ref_trace, brute force as in
+    its -s mode) on the first jobs of the stream, one per host core up to
+    `max_cores`, each for about budget_s / jobs seconds of loop (moves
+    estimated from its sites): run one after another on one thread (the
+    baseline of record), then all at once, one process per job (jobs are
+    independent boxes; aggregate = all moves / the longest loop).  Falls back
+    to the oracle restatement, single thread only, when the reference build
+    is absent."""
     import mcgpu_amd as m
     ref = ROOT / "oracle" / "_ref" / "ref_trace"
-    moves, secs, used = 0, 0.0, 0
-    per_job = max(1.0, budget_s / 6)
     kind = "reference" if ref.exists() else "port"
-    for j in jobs:
-        box = pool.get(j["kind"], j["n"])
-        # estimate moves that fit in per_job seconds (brute force ~ N^1)
-        steps = int(max(20, min(j["steps"], 4e6 / j["n"] * per_job / 2)))
-        with tempfile.TemporaryDirectory() as td:
-            p = Path(td) / "b.state"
+    cores = max(1, min(max_cores, os.cpu_count() or 1))
+    sample = jobs[:cores]
+    per_job = budget_s / len(sample)
+    with tempfile.TemporaryDirectory() as td:
+        runs = []
+        for k, j in enumerate(sample):
+            box = pool.get(j["kind"], j["n"])
+            d = box.desc()
+            # brute force costs ~ N x sites per move: ~4e6 / N moves/s for
+            # 3-site water on one core
+            rate = 1.2e7 / (d.n_molecules * (d.n_sites / d.n_molecules))
+            steps = int(max(20, min(j["steps"], rate * per_job)))
+            p = Path(td) / f"b{k}.state"
             box.write_state(str(p))
+            runs.append((j, steps, p))
+        moves = sum(st for _, st, _ in runs)
+        secs = 0.0
+        for j, steps, p in runs:
             if kind == "reference":
                 out = subprocess.run([str(ref), str(p), str(steps), "-"],
                                      capture_output=True, text=True, check=True).stdout
-                loop = float(re.search(r"Loop Seconds: ([0-9.eE+-]+)", out).group(1))
+                secs += _ref_loop_seconds(out)
             else:
                 from oracle import oracle as O
-                hb = m.HostBox.from_state(str(p))
-                ob = O.OracleBox(hb.arrays())
+                ob = O.OracleBox(m.HostBox.from_state(str(p)).arrays())
                 t = time.perf_counter()
                 ob.run_nvt(j["mc_seed"], steps, use_libc=True)
-                loop = time.perf_counter() - t
-        moves += steps
-        secs += loop
-        used += 1
-        if secs >= budget_s:
-            break
-    return dict(value=moves / secs, unit="moves/s", cores=1, kind=kind,
-                sample=(f"first {used} jobs of the stream, {moves} moves in total "
-                        f"(each job capped to ~{per_job:.0f} s), brute force, "
-                        f"single thread, {secs:.1f} s of loop time"))
-
-
-def cpu_baseline_all_cores(jobs, pool, budget_s: float, max_cores: int = 16):
-    """The reference's CPU path with one job per core, all at once (jobs are
-    independent boxes): the first `cores` jobs of the stream, each for
-    about budget_s / 2 seconds of loop; aggregate = sum of per-process move
-    rates while all run together."""
-    ref = ROOT / "oracle" / "_ref" / "ref_trace"
-    if not ref.exists():
-        return None
-    cores = max(1, min(max_cores, os.cpu_count() or 1))
-    per_job = max(1.0, budget_s / 2)
-    procs = []
-    with tempfile.TemporaryDirectory() as td:
-        for k, j in enumerate(jobs[:cores]):
-            box = pool.get(j["kind"], j["n"])
-            steps = int(max(20, min(j["steps"], 4e6 / j["n"] * per_job / 2)))
-            p = Path(td) / f"b{k}.state"
-            box.write_state(str(p))
-            procs.append((steps, subprocess.Popen([str(ref), str(p), str(steps), "-"],
-                                                  stdout=subprocess.PIPE, text=True)))
-        rate, moves = 0.0, 0
-        for steps, pr in procs:
-            out, _ = pr.communicate()
-            if pr.returncode != 0:
-                return None
-            loop = float(re.search(r"Loop Seconds: ([0-9.eE+-]+)", out).group(1))
-            rate += steps / loop
-            moves += steps
-    return dict(value=rate, unit="moves/s", cores=len(procs), kind="reference",
-                sample=(f"first {len(procs)} jobs of the stream, one process each, "
-                        f"{moves} moves in total (each capped to ~{per_job:.0f} s), "
-                        "brute force"))
+                secs += time.perf_counter() - t
+        out = dict(value=moves / secs, unit="moves/s", cores=1, kind=kind,
+                   sample=(f"first {len(runs)} jobs of the stream, {moves} moves in "
+                           f"total (each capped to ~{per_job:.1f} s of loop), brute "
+                           f"force, single thread, {secs:.1f} s of loop time"))
+        if kind == "reference" and len(runs) > 1:
+            procs = [subprocess.Popen([str(ref), str(p), str(steps), "-"],
+                                      stdout=subprocess.PIPE, text=True)
+                     for _, steps, p in runs]
+            loops = []
+            for pr in procs:
+                o, _ = pr.communicate()
+                if pr.returncode != 0:
+                    raise RuntimeError("reference CPU run failed")
+                loops.append(_ref_loop_seconds(o))
+            out["all_cores"] = dict(
+                value=moves / max(loops), unit="moves/s", cores=len(runs), kind=kind,
+                sample=(f"the same {len(runs)} jobs and moves, one process per job, all "
+                        f"at once; the longest loop took {max(loops):.1f} s"))
+    return out
 
 
 def cpu_model() -> str:
@@ -523,7 +517,6 @@ def main() -> int:
         cpu = cpu_baseline(timed, pool, args.cpu_seconds)
         cpu["cpu_model"] = cpu_model()
         cpu["cores_available"] = os.cpu_count()
-        cpu["all_cores"] = cpu_baseline_all_cores(timed, pool, args.cpu_seconds)
         print(f"# CPU baseline: {json.dumps(cpu)}", file=sys.stderr, flush=True)
 
     line = {
