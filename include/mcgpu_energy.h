@@ -142,6 +142,17 @@ int mcgpu_system_energy(mcgpu_box* box, double* energy);
 /* Number of real site pairs summed by the last mcgpu_system_energy call. */
 int mcgpu_system_pair_count(mcgpu_box* box, int64_t* pairs);
 
+/* mcgpu_system_energy without allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * kernels and result as mcgpu_system_energy, with the per-molecule partials
+ * and totals in a fixed device array of the library (2 MiB per GPU, held
+ * until the process ends) instead of the box's buffers; the call makes no
+ * hipMalloc/hipFree.  Boxes of more than 131,072 molecules are refused with
+ * MCGPU_ERR_RANGE.  Calls that overlap on one GPU share that array and race:
+ * serialise them.  Synchronous. */
+int mcgpu_system_energy_noalloc(mcgpu_box* box, double* energy);
+
 /* One trial move, fused: evaluates E_mol(0) at the current (old) and at the
  * proposed coordinates in one pass, applies the Metropolis test of
  * SimulationStep.cpp:794-803 with the caller's uniform draw u (used only when

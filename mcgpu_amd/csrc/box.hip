@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1030,6 +1031,83 @@ extern "C" int mcgpu_system_energy(mcgpu_box* b, double* energy) {
 extern "C" int mcgpu_system_pair_count(mcgpu_box* b, int64_t* pairs) {
   if (!b || !pairs) return MCGPU_ERR_INVALID;
   *pairs = b->last_pairs;
+  return MCGPU_OK;
+}
+
+// ---------------------------------------- full energy without allocation
+//
+// The per-molecule partial energies and pair counts of a full-energy pass,
+// and its two totals, live in one file-scope device array instead of the
+// box's own buffers: [0, K) partials, [K, 2K) pair counts, [2K, 2K + 2)
+// totals, K = kScratchMolecules.  Every entry a pass reads it writes first
+// (one wave per molecule writes that molecule's partial and count; the
+// reduction writes both totals), so nothing needs clearing between calls.
+// Each GPU holds its own copy until the process ends; calls that overlap on
+// one GPU share it and race.
+constexpr int kScratchMolecules = 1 << 17;
+static __device__ __align__(256) double scratch[2 * kScratchMolecules + 2];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+double* g_scratch[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// The device address of the current device's `scratch` (the name itself is
+// only a host-side stand-in): looked up once per device with that device
+// current, then reused.
+int scratch_address(double** out) {
+  int dev = 0;
+  MCGPU_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    set_error("device index beyond the scratch table");
+    return MCGPU_ERR_RANGE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return MCGPU_ERR_HIP;
+    }
+    g_scratch[dev] = (double*)p;
+  }
+  *out = g_scratch[dev];
+  return MCGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int mcgpu_system_energy_noalloc(mcgpu_box* b, double* energy) {
+  if (!b || !energy) return MCGPU_ERR_INVALID;
+  const int n = b->bv.n_mol;
+  if (n > kScratchMolecules) {
+    set_error("mcgpu_system_energy_noalloc: " + std::to_string(n) +
+              " molecules exceed the scratch limit of " + std::to_string(kScratchMolecules));
+    return MCGPU_ERR_RANGE;
+  }
+  double* ptr = nullptr;
+  int rc = scratch_address(&ptr);
+  if (rc) return rc;
+  double* partial = ptr;
+  double* pairs = ptr + kScratchMolecules;
+  double* total = ptr + 2 * kScratchMolecules;
+  const int grid = (n + 3) / 4;
+  if (b->precision == MCGPU_MIXED)
+    hipLaunchKernelGGL(k_system_waves<true>, dim3(grid), dim3(256), 0, b->stream, b->bv,
+                       partial, pairs);
+  else
+    hipLaunchKernelGGL(k_system_waves<false>, dim3(grid), dim3(256), 0, b->stream, b->bv,
+                       partial, pairs);
+  MCGPU_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, b->stream, partial, pairs, n, total);
+  MCGPU_HIP_TRY(hipGetLastError());
+  double t[2] = {0.0, 0.0};
+  MCGPU_HIP_TRY(hipMemcpyAsync(t, total, sizeof(t), hipMemcpyDeviceToHost, b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  *energy = t[0];
+  b->last_pairs = (int64_t)t[1];
   return MCGPU_OK;
 }
 
