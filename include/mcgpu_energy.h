@@ -232,6 +232,9 @@ typedef struct mcgpu_batch mcgpu_batch;
 int mcgpu_batch_create(int32_t max_boxes, size_t hbm_budget_bytes,
                        mcgpu_batch** out);
 int mcgpu_batch_destroy(mcgpu_batch* b);
+/* Makes the batch's device (the one current at mcgpu_batch_create) current
+ * on the calling thread, for a batch driven from another host thread. */
+int mcgpu_batch_bind_thread(mcgpu_batch* b);
 /* Device bytes a box would take (for admission decisions). */
 int mcgpu_box_footprint(const mcgpu_box_desc* desc,
                         const mcgpu_run_params* params, size_t* bytes);

@@ -149,43 +149,73 @@ typedef struct mcgpu_server_config {
   double tick_budget_us;  /* per-launch time limit (0 = none) */
 } mcgpu_server_config;
 
+/* Job states (mcgpu_job_result.state). */
+#define MCGPU_JOB_QUEUED 0  /* submitted, not admitted yet */
+#define MCGPU_JOB_RUNNING 1 /* resident on the device */
+#define MCGPU_JOB_DONE 2    /* finished (status 0) or failed (status < 0) */
+
 typedef struct mcgpu_job_result {
   int64_t job_id;
   double arrival_s;
   double admit_s;
   double finish_s;
-  double final_energy;
+  double final_energy; /* running energy (live while running) */
   double start_energy;
-  int64_t steps;
+  int64_t steps;       /* requested moves; moves made once done */
   int64_t accepted;
   int32_t status; /* 0 ok, <0 error code */
-  int32_t reserved;
+  int32_t state;  /* MCGPU_JOB_* */
+  int64_t steps_done;  /* moves made so far (updated after every tick) */
 } mcgpu_job_result;
 
+/* The serving loop runs on a host thread of the server, which alone drives
+ * the device; every call below may come from any thread, also while the
+ * loop runs.  The server clock starts at mcgpu_server_start. */
 typedef struct mcgpu_server mcgpu_server;
 int mcgpu_server_create(const mcgpu_server_config* cfg, mcgpu_server** out);
+/* Stops the loop (see mcgpu_server_stop) and frees the server. */
 int mcgpu_server_destroy(mcgpu_server* s);
-/* Queues a job.  The server reads `box` until mcgpu_server_run returns
- * (the caller keeps it alive).  arrival_s: when the job becomes visible, on
- * the server clock (seconds since mcgpu_server_run started). */
+/* Queues a job, before or after the start.  arrival_s: when the job becomes
+ * visible on the server clock; < 0 means now.  The server reads `box` when
+ * it admits the job: the caller keeps it alive until the job's state is no
+ * longer MCGPU_JOB_QUEUED.  Refused (MCGPU_ERR_STATE) once stop was called. */
 int mcgpu_server_submit(mcgpu_server* s, mcgpu_hostbox* box,
                         const mcgpu_run_params* params, double arrival_s,
                         int64_t* job_id);
-/* Starts the clock and runs the continuous-batching loop until every
- * submitted job has finished. */
+/* Starts the clock and the continuous-batching loop on the server's thread;
+ * returns at once. */
+int mcgpu_server_start(mcgpu_server* s);
+/* Blocks until job job_id (every submitted job when job_id < 0) is done,
+ * or timeout_s passes (MCGPU_ERR_STATE; < 0 waits without limit).  Returns
+ * the loop's error if the loop ended on one. */
+int mcgpu_server_wait(mcgpu_server* s, int64_t job_id, double timeout_s);
+/* Refuses further submissions, lets every submitted job finish, and joins
+ * the loop; returns the loop's status. */
+int mcgpu_server_stop(mcgpu_server* s);
+/* Start, wait for every submitted job, stop: a replay of a job list whose
+ * arrival times were given up front. */
 int mcgpu_server_run(mcgpu_server* s);
+/* One job's record as it stands (live progress while it runs). */
+int mcgpu_server_poll(mcgpu_server* s, int64_t job_id, mcgpu_job_result* out);
 int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out, int64_t cap,
                          int64_t* n);
 typedef struct mcgpu_server_stats {
-  double wall_s;
-  int64_t total_moves;
+  double wall_s;           /* server clock at the last completion */
+  int64_t total_moves;     /* moves made so far by every job */
   int64_t ticks;
-  double kernel_ms_total;
+  double kernel_ms_total;  /* summed kernel time (complete once stopped) */
   int64_t peak_resident;
-  size_t peak_bytes;
+  size_t peak_bytes;       /* arena high-water mark (set once stopped) */
   int64_t work_candidates; /* summed over retired jobs (see mcgpu_box_stats) */
   int64_t work_sites;
+  int32_t resident;        /* jobs on the device now */
+  int32_t reserved;
+  int64_t queued;          /* submitted, not admitted */
+  int64_t completed;
+  int64_t submitted;
+  double clock_s;          /* server clock now */
 } mcgpu_server_stats;
+/* Live statistics (any thread, any time). */
 int mcgpu_server_get_stats(mcgpu_server* s, mcgpu_server_stats* out);
 
 #ifdef __cplusplus

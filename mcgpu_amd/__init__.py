@@ -22,6 +22,9 @@ NEIGHBOR_CELL = 1
 NEIGHBOR_VERLET = 2
 NVT = 0
 NPT = 1
+JOB_QUEUED = 0
+JOB_RUNNING = 1
+JOB_DONE = 2
 WL_WATER = 0
 WL_BICYCLIC = 1
 WL_MIXTURE = 2
@@ -120,7 +123,8 @@ class JobResult(C.Structure):
         ("job_id", C.c_int64), ("arrival_s", C.c_double), ("admit_s", C.c_double),
         ("finish_s", C.c_double), ("final_energy", C.c_double),
         ("start_energy", C.c_double), ("steps", C.c_int64),
-        ("accepted", C.c_int64), ("status", C.c_int32), ("reserved", C.c_int32),
+        ("accepted", C.c_int64), ("status", C.c_int32), ("state", C.c_int32),
+        ("steps_done", C.c_int64),
     ]
 
 
@@ -129,7 +133,9 @@ class ServerStats(C.Structure):
         ("wall_s", C.c_double), ("total_moves", C.c_int64), ("ticks", C.c_int64),
         ("kernel_ms_total", C.c_double), ("peak_resident", C.c_int64),
         ("peak_bytes", C.c_size_t), ("work_candidates", C.c_int64),
-        ("work_sites", C.c_int64),
+        ("work_sites", C.c_int64), ("resident", C.c_int32), ("reserved", C.c_int32),
+        ("queued", C.c_int64), ("completed", C.c_int64), ("submitted", C.c_int64),
+        ("clock_s", C.c_double),
     ]
 
 

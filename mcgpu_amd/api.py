@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -458,8 +459,8 @@ class Server:
         self._s = C.c_void_p()
         check(lib().mcgpu_server_create(C.byref(cfg), C.byref(self._s)),
               "mcgpu_server_create")
-        self.n = 0
         self._boxes = []
+        self._lock = threading.Lock()
 
     def close(self):
         if self._s:
@@ -472,29 +473,56 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, box: HostBox, arrival_s: float, *, steps: int,
+    def submit(self, box: HostBox, arrival_s: float = -1.0, *, steps: int,
                precision: int = 0, ensemble: int = 0, volume_interval: int = 0,
                pressure: float = 0.0, max_volume_delta: float = 0.0,
                neighbor: int = 0, seed: int | None = None) -> int:
+        """Queues a job (any thread, before or while the loop runs);
+        arrival_s < 0 means now on the server clock."""
         d = box.desc()
         p = RunParams(box.mc_seed if seed is None else seed, precision, ensemble,
                       volume_interval, steps, pressure, max_volume_delta,
                       lrc_for_run(d, ensemble), neighbor, 0)
         jid = C.c_int64()
-        self._boxes.append(box)   # the server reads it until run() returns
+        with self._lock:
+            self._boxes.append(box)   # the server reads it at admission
         check(lib().mcgpu_server_submit(self._s, box.handle, C.byref(p),
                                         C.c_double(arrival_s), C.byref(jid)),
               "mcgpu_server_submit")
-        self.n += 1
         return jid.value
 
+    @property
+    def n(self) -> int:
+        return int(self.stats().submitted)
+
+    def start(self) -> None:
+        """Starts the loop on the server's own thread; returns at once."""
+        check(lib().mcgpu_server_start(self._s), "mcgpu_server_start")
+
+    def wait(self, job_id: int = -1, timeout_s: float = -1.0) -> None:
+        """Blocks until the job (every submitted job for -1) is done."""
+        check(lib().mcgpu_server_wait(self._s, C.c_int64(job_id), C.c_double(timeout_s)),
+              "mcgpu_server_wait")
+
+    def stop(self) -> None:
+        """Refuses new jobs, finishes the submitted ones, joins the loop."""
+        check(lib().mcgpu_server_stop(self._s), "mcgpu_server_stop")
+
+    def poll(self, job_id: int) -> JobResult:
+        r = JobResult()
+        check(lib().mcgpu_server_poll(self._s, C.c_int64(job_id), C.byref(r)),
+              "mcgpu_server_poll")
+        return r
+
     def run(self) -> None:
+        """Start, wait for every submitted job, stop."""
         check(lib().mcgpu_server_run(self._s), "mcgpu_server_run")
 
     def results(self) -> list:
-        out = (JobResult * max(self.n, 1))()
+        n_sub = self.n
+        out = (JobResult * max(n_sub, 1))()
         n = C.c_int64()
-        check(lib().mcgpu_server_results(self._s, out, C.c_int64(self.n), C.byref(n)))
+        check(lib().mcgpu_server_results(self._s, out, C.c_int64(n_sub), C.byref(n)))
         return [out[i] for i in range(n.value)]
 
     def stats(self) -> ServerStats:

@@ -865,6 +865,7 @@ struct mcgpu_batch {
   hipEvent_t ev_vol = nullptr;    // end of the last queued volume pass
   Arena arena;
   int32_t max_boxes = 0;
+  int device = 0;  // the device current when the batch was created
   std::vector<BatchSlot> slots;
   BatchBox* d_table = nullptr;
   BatchBox* h_table = nullptr;  // pinned mirror (the host's copy)
@@ -1174,6 +1175,7 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
   auto* b = new (std::nothrow) mcgpu_batch();
   if (!b) return MCGPU_ERR_NOMEM;
   b->max_boxes = max_boxes;
+  if (hipGetDevice(&b->device) != hipSuccess) b->device = 0;
   b->slots.resize(max_boxes);
   if (budget == 0) {
     size_t fr = 0, tot = 0;
@@ -1269,6 +1271,12 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   for (hipStream_t s : b->side)
     if (s) (void)hipStreamDestroy(s);
   delete b;
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_bind_thread(mcgpu_batch* b) {
+  if (!b) return MCGPU_ERR_INVALID;
+  MCGPU_HIP_TRY(hipSetDevice(b->device));
   return MCGPU_OK;
 }
 

@@ -2,18 +2,26 @@
 //
 // Jobs arrive over time (arrival_s on the server clock).  Between persistent
 // launches ("ticks") the loop admits every arrived job whose device
-// footprint fits the HBM budget and a free slot (FIFO), then launches one
-// tick that advances every resident box by up to sweeps_per_tick moves, then
-// retires finished boxes, freeing their slot and memory for the next
-// admissions.  Box state and RNG stay on the device from admission to
-// retirement.  Latency of a job = finish time - arrival time.
+// footprint fits the HBM budget and a free slot (FIFO by arrival), then
+// launches one tick that advances every resident box by up to
+// sweeps_per_tick moves, then retires finished boxes, freeing their slot and
+// memory for the next admissions.  Box state and RNG stay on the device from
+// admission to retirement.  Latency of a job = finish time - arrival time.
+//
+// The loop runs on a host thread of the server (mcgpu_server_start), which
+// alone drives the device batch.  Other threads submit jobs, poll them, read
+// live statistics and wait for completions at any time; one mutex guards
+// the job table and the statistics, and a condition variable wakes the loop
+// on a submission and waiters on a completion.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
-#include <string>
 #include <string>
 #include <thread>
 #include <vector>
@@ -25,26 +33,244 @@ namespace mcgpu {
 HostBox& hostbox_ref(mcgpu_hostbox* h);
 }
 
-using mcgpu::HostBox;
-
 namespace {
 
 struct Job {
-  mcgpu_hostbox* box;  // owned by the caller until mcgpu_server_run returns
+  mcgpu_hostbox* box;  // the caller's, read at admission
   mcgpu_run_params params;
   mcgpu_job_result res;
   int32_t slot = -1;
 };
 
+using Clock = std::chrono::steady_clock;
+
 }  // namespace
 
 struct mcgpu_server {
   mcgpu_server_config cfg{};
-  mcgpu_batch* batch = nullptr;
+  mcgpu_batch* batch = nullptr;  // driven by the loop thread only
+  std::mutex mu;                 // guards everything below
+  std::condition_variable cv_loop;  // submissions and stop wake the loop
+  std::condition_variable cv_done;  // completions wake waiters
   std::vector<Job> jobs;
+  std::vector<int64_t> pending;  // queued job ids, in arrival order
   mcgpu_server_stats stats{};
-  bool ran = false;
+  int64_t completed = 0;
+  int32_t resident_now = 0;
+  bool started = false, stopping = false, finished = false;
+  int loop_rc = MCGPU_OK;
+  std::string loop_error;
+  Clock::time_point t0;
+  std::thread loop;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
+
+namespace {
+
+// Queues job id k in arrival order (stable for equal arrivals).
+void enqueue_pending(mcgpu_server* s, int64_t k) {
+  const double a = s->jobs[k].res.arrival_s;
+  auto it = std::upper_bound(s->pending.begin(), s->pending.end(), a,
+                             [&](double t, int64_t j) { return t < s->jobs[j].res.arrival_s; });
+  s->pending.insert(it, k);
+}
+
+// The continuous-batching loop (runs on the server's thread).
+int serve_loop(mcgpu_server* s) {
+  std::vector<int64_t> resident;  // job ids on the device
+  mcgpu_box_stats st;
+  // MCGPU_SERVER_LOG=<seconds>: a progress line on stderr at that period
+  const char* log_env = std::getenv("MCGPU_SERVER_LOG");
+  const double log_every = log_env ? std::atof(log_env) : 0.0;
+  double next_log = log_every;
+  // No-progress watchdog: with jobs resident or already arrived, a tick or an
+  // admission must complete within this many seconds.
+  const double kStallLimit = 60.0;
+  double last_progress = 0.0;
+  double t_admit = 0.0, t_run = 0.0, t_retire = 0.0;  // host time per phase
+  double kernel_ms = 0.0;  // device time of the ticks (HIP events), summed
+  int64_t peak = 0;
+
+  // Admits arrived jobs while they fit (FIFO; the head waits for memory
+  // when boxes are resident).  Returns MCGPU_OK or a loop-ending error.
+  auto admit = [&](bool allow_empty_retry) {
+    for (;;) {
+      int64_t k;
+      mcgpu_box_desc d;
+      mcgpu_run_params p;
+      {
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (s->pending.empty() || s->jobs[s->pending.front()].res.arrival_s > s->now()) break;
+        k = s->pending.front();
+        d = mcgpu::hostbox_ref(s->jobs[k].box).desc();
+        p = s->jobs[k].params;
+      }
+      int32_t slot = -1;
+      const double ta = s->now();
+      const int rc = mcgpu_batch_add(s->batch, &d, &p, &slot);
+      if (log_every > 0 && s->now() - ta > log_every) {
+        std::fprintf(stderr, "[server] slow admission: job %lld (%d molecules) took %.2fs rc=%d\n",
+                     (long long)k, d.n_molecules, s->now() - ta, rc);
+        std::fflush(stderr);
+      }
+      if (rc == MCGPU_ERR_NOMEM && (!resident.empty() || !allow_empty_retry)) break;
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->pending.erase(s->pending.begin());
+      Job& j = s->jobs[k];
+      j.res.admit_s = s->now();
+      if (rc != MCGPU_OK) {  // failed job: isolated, reported, not retried
+        j.res.status = rc;
+        j.res.finish_s = j.res.admit_s;
+        j.res.state = MCGPU_JOB_DONE;
+        s->completed++;
+        s->cv_done.notify_all();
+        continue;
+      }
+      j.slot = slot;
+      j.res.state = MCGPU_JOB_RUNNING;
+      last_progress = j.res.admit_s;
+      resident.push_back(k);
+      s->resident_now = (int32_t)resident.size();
+    }
+  };
+
+  for (;;) {
+    const double t = s->now();
+    admit(true);
+    t_admit += s->now() - t;
+    peak = std::max<int64_t>(peak, (int64_t)resident.size());
+    bool arrived, stop_now;
+    double next_arrival = -1.0;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      arrived = !s->pending.empty() && s->jobs[s->pending.front()].res.arrival_s <= s->now();
+      if (!s->pending.empty()) next_arrival = s->jobs[s->pending.front()].res.arrival_s;
+      stop_now = s->stopping && s->pending.empty() && resident.empty();
+      s->stats.peak_resident = peak;
+    }
+    if (stop_now) break;
+    if ((arrived || !resident.empty()) && s->now() - last_progress > kStallLimit) {
+      mcgpu::set_error("mcgpu_server: no progress for 60 s (" + std::to_string(resident.size()) +
+                       " resident)");
+      return MCGPU_ERR_STATE;
+    }
+    if (resident.empty()) {
+      if (!arrived) last_progress = s->now();
+      // idle: sleep until the next arrival, a submission or stop
+      std::unique_lock<std::mutex> lock(s->mu);
+      const double wait = next_arrival < 0 ? 0.05 : std::min(0.05, next_arrival - s->now());
+      if (wait > 0 && !s->stopping)
+        s->cv_loop.wait_for(lock, std::chrono::duration<double>(wait));
+      continue;
+    }
+    // The tick runs asynchronously: jobs that arrive meanwhile are admitted
+    // (host set-up and queued uploads) while it runs; retirement below
+    // waits for it.
+    const double tt = s->now();
+    int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 0);
+    if (rc != MCGPU_OK) return rc;
+    t_run += s->now() - tt;
+    {
+      const double t2 = s->now();
+      admit(false);
+      t_admit += s->now() - t2;
+    }
+    last_progress = s->now();
+    if (log_every > 0 && s->now() - tt > log_every) {
+      std::fprintf(stderr, "[server] slow tick: %.2fs with %zu resident\n", s->now() - tt,
+                   resident.size());
+      std::fflush(stderr);
+    }
+    if (log_every > 0 && s->now() >= next_log) {
+      double kms = 0.0;
+      int64_t nl = 0;
+      if (mcgpu_batch_kernel_ms(s->batch, &kms, &nl) == MCGPU_OK) kernel_ms += kms * nl;
+      double tw = 0, tv = 0, tl = 0;
+      (void)mcgpu_batch_host_seconds(s->batch, &tw, &tv, &tl);
+      std::lock_guard<std::mutex> lock(s->mu);
+      std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu pending=%zu done=%lld moves=%lld "
+                   "host s: admit %.2f run %.2f (kernels %.2f; wait %.2f volume %.2f launch %.2f) "
+                   "retire %.2f\n",
+                   s->now(), (long long)s->stats.ticks + 1, resident.size(), s->pending.size(),
+                   (long long)s->completed, (long long)s->stats.total_moves, t_admit, t_run,
+                   kernel_ms / 1e3, tw, tv, tl, t_retire);
+      std::fflush(stderr);
+      next_log = s->now() + log_every;
+    }
+    // live progress of every resident job, and retirement of finished ones
+    const double tr = s->now();
+    int64_t moved = 0;
+    for (size_t k = 0; k < resident.size();) {
+      const int64_t id = resident[k];
+      int32_t slot;
+      {
+        std::lock_guard<std::mutex> lock(s->mu);
+        slot = s->jobs[id].slot;
+      }
+      rc = mcgpu_batch_stats(s->batch, slot, &st);
+      if (rc != MCGPU_OK) return rc;
+      std::unique_lock<std::mutex> lock(s->mu);
+      Job& j = s->jobs[id];
+      moved += st.steps_done - j.res.steps_done;
+      j.res.steps_done = st.steps_done;
+      j.res.accepted = st.accepted;
+      j.res.final_energy = st.energy;
+      j.res.start_energy = st.start_energy;
+      if (!st.done) {
+        k++;
+        continue;
+      }
+      j.res.finish_s = s->now();
+      j.res.steps = st.steps_done;
+      j.res.status = st.fault ? MCGPU_ERR_STATE : MCGPU_OK;
+      s->stats.work_candidates += st.work_candidates;
+      s->stats.work_sites += st.work_sites;
+      lock.unlock();
+      if (st.fault && log_every > 0) {
+        std::fprintf(stderr, "[server] job %lld stopped by device fault %d after %lld moves\n",
+                     (long long)id, st.fault, (long long)st.steps_done);
+        std::fflush(stderr);
+      }
+      rc = mcgpu_batch_remove(s->batch, slot);
+      if (rc != MCGPU_OK) return rc;
+      lock.lock();
+      j.slot = -1;
+      j.res.state = MCGPU_JOB_DONE;
+      s->completed++;
+      s->stats.wall_s = j.res.finish_s;
+      resident[k] = resident.back();
+      resident.pop_back();
+      s->resident_now = (int32_t)resident.size();
+      s->cv_done.notify_all();
+    }
+    t_retire += s->now() - tr;
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->stats.ticks++;
+    s->stats.total_moves += moved;
+  }
+  size_t used = 0, pk = 0, capb = 0;
+  double avg_ms = 0;
+  int64_t launches = 0;
+  if (mcgpu_batch_kernel_ms(s->batch, &avg_ms, &launches) == MCGPU_OK)
+    kernel_ms += avg_ms * launches;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (mcgpu_batch_memory(s->batch, &used, &pk, &capb) == MCGPU_OK) s->stats.peak_bytes = pk;
+  s->stats.kernel_ms_total = kernel_ms;
+  return MCGPU_OK;
+}
+
+void loop_main(mcgpu_server* s) {
+  int rc = mcgpu_batch_bind_thread(s->batch);
+  if (rc == MCGPU_OK) rc = serve_loop(s);
+  std::lock_guard<std::mutex> lock(s->mu);
+  s->loop_rc = rc;
+  if (rc != MCGPU_OK) s->loop_error = mcgpu_last_error();
+  s->finished = true;
+  s->cv_done.notify_all();
+}
+
+}  // namespace
 
 extern "C" int mcgpu_server_create(const mcgpu_server_config* cfg,
                                    mcgpu_server** out) {
@@ -67,8 +293,23 @@ extern "C" int mcgpu_server_create(const mcgpu_server_config* cfg,
   return MCGPU_OK;
 }
 
+extern "C" int mcgpu_server_stop(mcgpu_server* s) {
+  if (!s) return MCGPU_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (!s->started) return MCGPU_OK;
+    s->stopping = true;
+  }
+  s->cv_loop.notify_all();
+  if (s->loop.joinable()) s->loop.join();
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (s->loop_rc != MCGPU_OK) mcgpu::set_error(s->loop_error);
+  return s->loop_rc;
+}
+
 extern "C" int mcgpu_server_destroy(mcgpu_server* s) {
   if (!s) return MCGPU_OK;
+  (void)mcgpu_server_stop(s);
   if (s->batch) mcgpu_batch_destroy(s->batch);
   delete s;
   return MCGPU_OK;
@@ -77,190 +318,103 @@ extern "C" int mcgpu_server_destroy(mcgpu_server* s) {
 extern "C" int mcgpu_server_submit(mcgpu_server* s, mcgpu_hostbox* box,
                                    const mcgpu_run_params* params,
                                    double arrival_s, int64_t* job_id) {
-  if (!s || !box || !params || s->ran) {
-    mcgpu::set_error("mcgpu_server_submit: bad argument or server already ran");
+  if (!s || !box || !params) {
+    mcgpu::set_error("mcgpu_server_submit: bad argument");
     return MCGPU_ERR_INVALID;
   }
-  Job j;
-  j.box = box;
-  j.params = *params;
-  std::memset(&j.res, 0, sizeof(j.res));
-  j.res.job_id = (int64_t)s->jobs.size();
-  j.res.arrival_s = arrival_s;
-  j.res.steps = params->steps;
-  s->jobs.push_back(j);
-  if (job_id) *job_id = j.res.job_id;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->stopping) {
+      mcgpu::set_error("mcgpu_server_submit: the server is stopping");
+      return MCGPU_ERR_STATE;
+    }
+    Job j;
+    j.box = box;
+    j.params = *params;
+    std::memset(&j.res, 0, sizeof(j.res));
+    j.res.job_id = (int64_t)s->jobs.size();
+    j.res.arrival_s = arrival_s >= 0 ? arrival_s : (s->started ? s->now() : 0.0);
+    j.res.steps = params->steps;
+    j.res.state = MCGPU_JOB_QUEUED;
+    s->jobs.push_back(j);
+    enqueue_pending(s, j.res.job_id);
+    if (job_id) *job_id = j.res.job_id;
+  }
+  s->cv_loop.notify_all();
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_start(mcgpu_server* s) {
+  if (!s) return MCGPU_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (s->started) {
+    mcgpu::set_error("mcgpu_server_start: already started");
+    return MCGPU_ERR_STATE;
+  }
+  s->started = true;
+  s->t0 = Clock::now();
+  try {
+    s->loop = std::thread(loop_main, s);
+  } catch (...) {
+    s->started = false;
+    mcgpu::set_error("mcgpu_server_start: cannot start the loop thread");
+    return MCGPU_ERR_NOMEM;
+  }
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_server_wait(mcgpu_server* s, int64_t job_id, double timeout_s) {
+  if (!s) return MCGPU_ERR_INVALID;
+  std::unique_lock<std::mutex> lock(s->mu);
+  if (job_id >= (int64_t)s->jobs.size()) {
+    mcgpu::set_error("mcgpu_server_wait: unknown job");
+    return MCGPU_ERR_RANGE;
+  }
+  if (!s->started) {
+    mcgpu::set_error("mcgpu_server_wait: the server has not started");
+    return MCGPU_ERR_STATE;
+  }
+  auto done = [&]() {
+    if (s->finished) return true;
+    return job_id >= 0 ? s->jobs[job_id].res.state == MCGPU_JOB_DONE
+                       : s->completed == (int64_t)s->jobs.size();
+  };
+  if (timeout_s < 0) {
+    s->cv_done.wait(lock, done);
+  } else if (!s->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), done)) {
+    mcgpu::set_error("mcgpu_server_wait: timed out");
+    return MCGPU_ERR_STATE;
+  }
+  if (s->finished && s->loop_rc != MCGPU_OK) {
+    mcgpu::set_error(s->loop_error);
+    return s->loop_rc;
+  }
   return MCGPU_OK;
 }
 
 extern "C" int mcgpu_server_run(mcgpu_server* s) {
-  if (!s || s->ran) return MCGPU_ERR_INVALID;
-  s->ran = true;
-  std::vector<size_t> order(s->jobs.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return s->jobs[a].res.arrival_s < s->jobs[b].res.arrival_s;
-  });
-  using clk = std::chrono::steady_clock;
-  const auto t0 = clk::now();
-  auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-  std::vector<int32_t> resident;  // job indices on the device
-  size_t next = 0, completed = 0;
-  int64_t peak = 0;
-  mcgpu_box_stats st;
-  // MCGPU_SERVER_LOG=<seconds>: a progress line on stderr at that period
-  const char* log_env = std::getenv("MCGPU_SERVER_LOG");
-  const double log_every = log_env ? std::atof(log_env) : 0.0;
-  double next_log = log_every;
-  // No-progress watchdog: with jobs resident or already arrived, a tick or an
-  // admission must complete within this many seconds.
-  const double kStallLimit = 60.0;
-  double last_progress = 0.0;
-  double t_admit = 0.0, t_run = 0.0, t_retire = 0.0;  // host time per phase
-  double kernel_ms = 0.0;  // device time of the ticks (HIP events), summed
-  while (completed < s->jobs.size()) {
-    // admit every arrived job that fits (FIFO; the head waits for memory)
-    const double t = now();
-    while (next < order.size() && s->jobs[order[next]].res.arrival_s <= t) {
-      Job& j = s->jobs[order[next]];
-      const mcgpu_box_desc d = mcgpu::hostbox_ref(j.box).desc();
-      int32_t slot = -1;
-      const double ta = now();
-      const int rc = mcgpu_batch_add(s->batch, &d, &j.params, &slot);
-      if (log_every > 0 && now() - ta > log_every) {
-        std::fprintf(stderr, "[server] slow admission: job %lld (%d molecules) took %.2fs rc=%d\n",
-                     (long long)j.res.job_id, d.n_molecules, now() - ta, rc);
-        std::fflush(stderr);
-      }
-      if (rc == MCGPU_ERR_NOMEM && !resident.empty()) break;  // wait for room
-      next++;
-      if (rc != MCGPU_OK) {  // failed job: isolated, reported, not retried
-        j.res.status = rc;
-        j.res.admit_s = j.res.finish_s = now();
-        completed++;
-        continue;
-      }
-      j.slot = slot;
-      j.res.admit_s = now();
-      last_progress = j.res.admit_s;
-      resident.push_back((int32_t)order[next - 1]);
-    }
-    t_admit += now() - t;
-    peak = std::max<int64_t>(peak, (int64_t)resident.size());
-    const bool arrived = next < order.size() && s->jobs[order[next]].res.arrival_s <= now();
-    if ((arrived || !resident.empty()) && now() - last_progress > kStallLimit) {
-      mcgpu::set_error("mcgpu_server_run: no progress for 60 s (" +
-                       std::to_string(resident.size()) + " resident, " +
-                       std::to_string(completed) + " done)");
-      return MCGPU_ERR_STATE;
-    }
-    if (resident.empty()) {
-      if (!arrived) last_progress = now();
-      if (next < order.size()) {
-        const double wait = s->jobs[order[next]].res.arrival_s - now();
-        if (wait > 0)
-          std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
-      }
-      continue;
-    }
-    // The tick runs asynchronously: jobs that arrive meanwhile are admitted
-    // (host set-up and queued uploads) while it runs; retirement below
-    // waits for it.
-    const double tt = now();
-    int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 0);
-    if (rc != MCGPU_OK) return rc;
-    t_run += now() - tt;
-    {
-      const double t2 = now();
-      while (next < order.size() && s->jobs[order[next]].res.arrival_s <= t2) {
-        Job& j = s->jobs[order[next]];
-        const mcgpu_box_desc d = mcgpu::hostbox_ref(j.box).desc();
-        int32_t slot = -1;
-        rc = mcgpu_batch_add(s->batch, &d, &j.params, &slot);
-        if (rc == MCGPU_ERR_NOMEM) break;  // wait for room after retirement
-        next++;
-        if (rc != MCGPU_OK) {
-          j.res.status = rc;
-          j.res.admit_s = j.res.finish_s = now();
-          completed++;
-          continue;
-        }
-        j.slot = slot;
-        j.res.admit_s = now();
-        resident.push_back((int32_t)order[next - 1]);
-      }
-      t_admit += now() - t2;
-    }
-    s->stats.ticks++;
-    last_progress = now();
-    if (log_every > 0 && now() - tt > log_every) {
-      std::fprintf(stderr, "[server] slow tick %lld: %.2fs with %zu resident\n",
-                   (long long)s->stats.ticks, now() - tt, resident.size());
-      std::fflush(stderr);
-    }
-    if (log_every > 0 && now() >= next_log) {
-      double kms = 0.0;
-      int64_t nl = 0;
-      if (mcgpu_batch_kernel_ms(s->batch, &kms, &nl) == MCGPU_OK) kernel_ms += kms * nl;
-      double tw = 0, tv = 0, tl = 0;
-      (void)mcgpu_batch_host_seconds(s->batch, &tw, &tv, &tl);
-      std::fprintf(stderr, "[server] t=%.2fs ticks=%lld resident=%zu admitted=%zu done=%zu moves=%lld "
-                   "host s: admit %.2f run %.2f (kernels %.2f; wait %.2f volume %.2f launch %.2f) "
-                   "retire %.2f\n",
-                   now(), (long long)s->stats.ticks, resident.size(), next, completed,
-                   (long long)s->stats.total_moves, t_admit, t_run, kernel_ms / 1e3, tw, tv, tl,
-                   t_retire);
-      std::fflush(stderr);
-      next_log = now() + log_every;
-    }
-    // retire finished jobs
-    const double tr = now();
-    for (size_t k = 0; k < resident.size();) {
-      Job& j = s->jobs[resident[k]];
-      rc = mcgpu_batch_stats(s->batch, j.slot, &st);
-      if (rc != MCGPU_OK) return rc;
-      if (st.done) {
-        j.res.finish_s = now();
-        j.res.final_energy = st.energy;
-        j.res.start_energy = st.start_energy;
-        j.res.accepted = st.accepted;
-        j.res.steps = st.steps_done;
-        j.res.status = st.fault ? MCGPU_ERR_STATE : MCGPU_OK;
-        if (st.fault && log_every > 0) {
-          std::fprintf(stderr, "[server] job %lld stopped by device fault %d after %lld moves\n",
-                       (long long)j.res.job_id, st.fault, (long long)st.steps_done);
-          std::fflush(stderr);
-        }
-        s->stats.total_moves += st.steps_done;
-        s->stats.work_candidates += st.work_candidates;
-        s->stats.work_sites += st.work_sites;
-        rc = mcgpu_batch_remove(s->batch, j.slot);
-        if (rc != MCGPU_OK) return rc;
-        j.slot = -1;
-        completed++;
-        resident[k] = resident.back();
-        resident.pop_back();
-      } else {
-        k++;
-      }
-    }
-    t_retire += now() - tr;
+  if (!s) return MCGPU_ERR_INVALID;
+  int rc = mcgpu_server_start(s);
+  if (rc == MCGPU_OK) rc = mcgpu_server_wait(s, -1, -1.0);
+  const int rs = mcgpu_server_stop(s);
+  return rc != MCGPU_OK ? rc : rs;
+}
+
+extern "C" int mcgpu_server_poll(mcgpu_server* s, int64_t job_id, mcgpu_job_result* out) {
+  if (!s || !out) return MCGPU_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (job_id < 0 || job_id >= (int64_t)s->jobs.size()) {
+    mcgpu::set_error("mcgpu_server_poll: unknown job");
+    return MCGPU_ERR_RANGE;
   }
-  s->stats.wall_s = now();
-  s->stats.peak_resident = peak;
-  size_t used = 0, pk = 0, capb = 0;
-  if (mcgpu_batch_memory(s->batch, &used, &pk, &capb) == MCGPU_OK) s->stats.peak_bytes = pk;
-  double avg_ms = 0;
-  int64_t launches = 0;
-  if (mcgpu_batch_kernel_ms(s->batch, &avg_ms, &launches) == MCGPU_OK)
-    kernel_ms += avg_ms * launches;
-  s->stats.kernel_ms_total = kernel_ms;
+  *out = s->jobs[job_id].res;
   return MCGPU_OK;
 }
 
 extern "C" int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out,
                                     int64_t cap, int64_t* n) {
   if (!s || !n) return MCGPU_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(s->mu);
   const int64_t k = std::min<int64_t>(cap, (int64_t)s->jobs.size());
   for (int64_t i = 0; i < k; i++) out[i] = s->jobs[i].res;
   *n = k;
@@ -269,6 +423,12 @@ extern "C" int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out,
 
 extern "C" int mcgpu_server_get_stats(mcgpu_server* s, mcgpu_server_stats* out) {
   if (!s || !out) return MCGPU_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(s->mu);
   *out = s->stats;
+  out->resident = s->resident_now;
+  out->queued = (int64_t)s->pending.size();
+  out->completed = s->completed;
+  out->submitted = (int64_t)s->jobs.size();
+  out->clock_s = s->started ? s->now() : 0.0;
   return MCGPU_OK;
 }
