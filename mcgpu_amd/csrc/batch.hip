@@ -272,6 +272,8 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
     nitems = 0;
   }
   if (lane == 0) {
+    // a cell-list box whose move scans every molecule
+    if (bv.mode == MCGPU_NB_CELL && src != MCGPU_NB_CELL) S.c.fallbacks++;
     sh.src = src;
     sh.nitems = nitems;
     sh.ncells = ncells;
@@ -315,7 +317,6 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
     S.c.work_cand += (unsigned)(wc & 0xffffffffu);
     S.c.work_sites += (unsigned)(wc >> 32);
     S.c.work_self += 2 * len;
-    if (bv.mode == MCGPU_NB_CELL && sh.src != MCGPU_NB_CELL) S.c.fallbacks++;
     if (S.tb.record) {
       S.tb.tr_mol[step] = mol;
       S.tb.tr_de[step] = de;
@@ -812,7 +813,7 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
 constexpr int kGroups = 3;
 constexpr int kGroupNT[kGroups] = {256, 512, 1024};
 
-int choose_group(const mcgpu_box_desc& d, int neighbor, int max_len) {
+double move_work(const mcgpu_box_desc& d, int neighbor, int max_len) {
   const double V = d.box[0] * d.box[1] * d.box[2];
   const double rho = d.n_molecules / V;
   const double rc = d.cutoff;
@@ -821,9 +822,10 @@ int choose_group(const mcgpu_box_desc& d, int neighbor, int max_len) {
   if (neighbor == MCGPU_NEIGHBOR_CELL) cand = std::min<double>(cand, 27.0 * rc * rc * rc * rho);
   int aw = 1;
   while (aw < max_len) aw *= 2;
-  const double work = cand + hits * max_len * aw / 2;
-  return work > 1000.0 ? 2 : (work > 600.0 ? 1 : 0);
+  return cand + hits * max_len * aw / 2;
 }
+
+int choose_group(double work) { return work > 1000.0 ? 2 : (work > 600.0 ? 1 : 0); }
 
 }  // namespace mcgpu
 
@@ -835,7 +837,8 @@ struct BatchSlot {
   BoxLayout lay{};
   mcgpu_run_params params{};
   int32_t n_pidx = 0;
-  int32_t group = 0;
+  int32_t group = 0;   // preferred work-group size (estimate or tuner)
+  double work = 0.0;   // per-move work estimate (move_work)
   double spread = 0.0; // largest primary-to-first-primary distance
   int fault = 0;
   int64_t tune_key = 0;    // box class for the work-group-size tuner
@@ -866,6 +869,12 @@ struct mcgpu_batch {
   Arena arena;
   int32_t max_boxes = 0;
   int device = 0;  // the device current when the batch was created
+  // Compute units, and the share of one a resident work-group of each size
+  // takes (1 / the work-groups of that kernel one CU holds at once, from
+  // its registers and LDS): a tick whose boxes need more than the device
+  // holds runs its work-groups in rounds.
+  int n_cu = 256;
+  double cu_share[kGroups] = {1.0 / 3.0, 1.0, 1.0};
   std::vector<BatchSlot> slots;
   BatchBox* d_table = nullptr;
   BatchBox* h_table = nullptr;  // pinned mirror (the host's copy)
@@ -878,6 +887,7 @@ struct mcgpu_batch {
   int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
   int32_t* h_active = nullptr;  // pinned     [count, slots...]
   std::vector<int32_t> next_active;  // the lists of the tick being set up
+  std::vector<int> eff_group;        // work-group size of each box this tick
   VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
   VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
@@ -1187,6 +1197,22 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
     return code;
   };
   if ((rc = b->arena.init(budget))) return fail(rc);
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) ==
+            hipSuccess &&
+        cus > 0)
+      b->n_cu = cus;
+    const void* kern[kGroups] = {(const void*)k_batch<256>, (const void*)k_batch<512>,
+                                 (const void*)k_batch<1024>};
+    for (int g = 0; g < kGroups; g++) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern[g], kGroupNT[g], 0) ==
+              hipSuccess &&
+          per_cu > 0)
+        b->cu_share[g] = 1.0 / per_cu;
+    }
+  }
   int dev = 0, khz = 0;
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
@@ -1357,7 +1383,8 @@ static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_r
   S.params = *p;
   S.n_pidx = d->n_primary;
   S.spread = pt.spread;
-  S.group = choose_group(*d, p->neighbor, pt.max_len);
+  S.work = move_work(*d, p->neighbor, pt.max_len);
+  S.group = choose_group(S.work);
   S.tune_key = tune_key_of(*d, *p, pt.max_len);
   if (b->autotune && b->budget_ticks > 0) {
     auto& T = b->tune[S.tune_key];
@@ -1608,13 +1635,32 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   for (int s = 0; s < b->max_boxes; s++) {
     const BatchSlot& S = b->slots[s];
     if (S.used && b->h_ctl[s].status == kRunning) {
-      act[S.group * stride + 1 + n[S.group]++] = s;
       b->launched.push_back(s);
       b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
     }
   }
   total = (int)b->launched.size();
   if (total == 0) return MCGPU_OK;
+  // Work-group sizes of this tick: each box's preferred one, unless the
+  // boxes need more compute units than the device has.  Then every box
+  // takes 256 lanes, whose work-groups share CUs, so the whole tick is
+  // resident at once: with 512- and 1024-lane groups oversubscribed, groups
+  // wait for whole CUs and run in rounds (512 water boxes of 64-4,096
+  // molecules: 1.9e7 moves/s with the preferred sizes, 2.3e7 all at 256
+  // lanes; demoting only the lightest boxes until the CUs sufficed gave
+  // 1.8e7-2.4e7, as the three group kernels then fragment the CUs).  Boxes
+  // measured by the tuner keep their size.
+  std::vector<int>& eff = b->eff_group;
+  eff.assign(b->max_boxes, 0);
+  double demand = 0.0;
+  for (int s : b->launched) {
+    eff[s] = b->slots[s].group;
+    demand += b->cu_share[eff[s]];
+  }
+  if (demand > b->n_cu)
+    for (int s : b->launched)
+      if (b->slots[s].trial < 0) eff[s] = 0;
+  for (int s : b->launched) act[eff[s] * stride + 1 + n[eff[s]]++] = s;
   const double tl = host_now();
   // set-up work queued so far (admissions, decided volume moves) first
   MCGPU_HIP_TRY(hipEventRecord(b->ev_ready, b->sstream));
