@@ -838,7 +838,6 @@ struct BatchSlot {
   mcgpu_run_params params{};
   int32_t n_pidx = 0;
   int32_t group = 0;   // preferred work-group size (estimate or tuner)
-  double work = 0.0;   // per-move work estimate (move_work)
   double spread = 0.0; // largest primary-to-first-primary distance
   int fault = 0;
   int64_t tune_key = 0;    // box class for the work-group-size tuner
@@ -1383,8 +1382,7 @@ static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_r
   S.params = *p;
   S.n_pidx = d->n_primary;
   S.spread = pt.spread;
-  S.work = move_work(*d, p->neighbor, pt.max_len);
-  S.group = choose_group(S.work);
+  S.group = choose_group(move_work(*d, p->neighbor, pt.max_len));
   S.tune_key = tune_key_of(*d, *p, pt.max_len);
   if (b->autotune && b->budget_ticks > 0) {
     auto& T = b->tune[S.tune_key];
