@@ -105,6 +105,9 @@ struct BatchShared {
   int32_t mol, start, pstart;
   int32_t cells_live;  // cell lists in use and complete for this move
   int32_t co_cell, cn_cell;  // first-primary cell before / after the move
+  // the cell lists' overflow flag, and whether ev.ccount holds the cell
+  // occupancies, for this launch (only this launch's moves change them)
+  int32_t ovf, cc_live;
 };
 
 // Wave 0: draws and builds the next trial move (Simulation.cpp:219-264 draw
@@ -178,7 +181,7 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
     }
     npri = npri < 1 ? 1 : (npri > kMaxSites ? kMaxSites : npri);
   }
-  const int cells_live = bv.mode == MCGPU_NB_CELL && bv.cells.overflow[0] == 0;
+  const int cells_live = bv.mode == MCGPU_NB_CELL && S.ovf == 0;
   // committed stores of the previous move (other lanes' cell counts) are
   // complete before this move's loads
   __threadfence_block();
@@ -260,7 +263,8 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
     cold = cell_index(cg, co[0], co[1], co[2]);
     cnew = cell_index(cg, cn[0], cn[1], cn[2]);
     int ni = 0;
-    const int nc = prepare_cells_wave(bv, sh, co, cn, &ni);
+    // (the item -> cell map is filled by every wave after the barrier)
+    const int nc = prepare_cells_wave(bv, sh, co, cn, &ni, S.cc_live != 0, false);
     if (nc >= 0) {
       src = MCGPU_NB_CELL;
       nitems = ni;
@@ -345,10 +349,17 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
   // from the same positions), checked against the list before it is used
   if (lane == 0 && S.cells_live && S.co_cell != S.cn_cell) {
     const CellGrid& cg = bv.cells;
-    if (cg.mol_cell[mol] == S.co_cell)
-      cell_move(cg, mol, S.co_cell, S.cn_cell);
-    else
+    if (cg.mol_cell[mol] == S.co_cell) {
+      const bool placed = cell_move(cg, mol, S.co_cell, S.cn_cell);
+      if (S.cc_live) {  // the LDS occupancies follow the lists
+        S.ev.ccount[S.co_cell] -= 1;
+        if (placed) S.ev.ccount[S.cn_cell] += 1;
+      }
+      if (!placed) S.ovf = 1;
+    } else {
       cg.overflow[0] = 1;  // lists disagree with positions: scan everything
+      S.ovf = 1;
+    }
   }
 }
 
@@ -386,16 +397,31 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   const BoxView& bv = S.tb.bv;
   const bool mixed = S.tb.precision == MCGPU_MIXED;
   load_params<NT>(bv, S.ev);
+  // the cell occupancies into LDS, when the grid fits: only this launch's
+  // commits change them until it ends
+  const bool cell_box = bv.mode == MCGPU_NB_CELL;
+  const int ncell = cell_box ? bv.cells.n[0] * bv.cells.n[1] * bv.cells.n[2] : 0;
+  const bool cc = cell_box && ncell <= EvalShared<NT>::kCellLds;
+  if (cc)
+    for (int c = t; c < ncell; c += NT) S.ev.ccount[c] = bv.cells.count[c];
+  if (t == 0) {
+    S.cc_live = cc;
+    S.ovf = cell_box ? bv.cells.overflow[0] : 0;
+  }
   __syncthreads();
   for (int s = 0; s < sweeps; s++) {
     if (w0) front_move<NT>(S, g, s, budget, t_start);
     __syncthreads();
     if (!S.go || S.ev.fault) break;
+    if (__builtin_amdgcn_readfirstlane(S.ev.src) == MCGPU_NB_CELL) {
+      fill_item_cells<NT>(S.ev, __builtin_amdgcn_readfirstlane(S.ev.ncells));
+      __syncthreads();
+    }
     const int mol = __builtin_amdgcn_readfirstlane(S.mol);
     if (mixed)
-      eval_core<NT, true>(bv, S.ev, mol, 0, true);
+      eval_core<NT, true>(bv, S.ev, mol, 0, true, false);
     else
-      eval_core<NT, false>(bv, S.ev, mol, 0, true);
+      eval_core<NT, false>(bv, S.ev, mol, 0, true, false);
     if (w0) finish_move<NT>(S, g);
   }
   if (w0) {
@@ -1401,22 +1427,22 @@ static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_r
   BatchBox& E = b->h_table[s];
   E = BatchBox{};
   BoxView& v = E.bv;
-  v.x = (double*)(base + L.x);
-  v.y = (double*)(base + L.y);
-  v.z = (double*)(base + L.z);
-  v.px = (double*)(base + L.px);
-  v.py = (double*)(base + L.py);
-  v.pz = (double*)(base + L.pz);
-  v.mol_start = (int32_t*)(base + L.ms);
-  v.mol_len = (int32_t*)(base + L.ml);
-  v.mol_pidx_start = (int32_t*)(base + L.mps);
-  v.mol_pidx_count = (int32_t*)(base + L.mpc);
-  v.mol_param = (int32_t*)(base + L.mp);
-  v.meta = (MolMeta*)(base + L.meta);
+  v.x = gmem((double*)(base + L.x));
+  v.y = gmem((double*)(base + L.y));
+  v.z = gmem((double*)(base + L.z));
+  v.px = gmem((double*)(base + L.px));
+  v.py = gmem((double*)(base + L.py));
+  v.pz = gmem((double*)(base + L.pz));
+  v.mol_start = gmem((int32_t*)(base + L.ms));
+  v.mol_len = gmem((int32_t*)(base + L.ml));
+  v.mol_pidx_start = gmem((int32_t*)(base + L.mps));
+  v.mol_pidx_count = gmem((int32_t*)(base + L.mpc));
+  v.mol_param = gmem((int32_t*)(base + L.mp));
+  v.meta = gmem((MolMeta*)(base + L.meta));
   v.single_primary = pt.single_primary;
-  v.pidx = (int32_t*)(base + L.pidx);
-  v.p64 = (SiteParam64*)(base + L.p64);
-  v.p32 = (SiteParam32*)(base + L.p32);
+  v.pidx = gmem((int32_t*)(base + L.pidx));
+  v.p64 = gmem((SiteParam64*)(base + L.p64));
+  v.p32 = gmem((SiteParam32*)(base + L.p32));
   v.n_mol = d->n_molecules;
   v.n_site = d->n_sites;
   v.max_len = pt.max_len;
@@ -1429,14 +1455,14 @@ static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_r
   v.cutoff2 = d->cutoff * d->cutoff;
   v.reach2 = reach2_of(pt.reach);
   v.kT = d->kT;
-  v.cells.count = (int32_t*)(base + L.ccount);
-  v.cells.slots = (int32_t*)(base + L.cslots);
-  v.cells.mol_cell = (int32_t*)(base + L.mcell);
-  v.cells.mol_slot = (int32_t*)(base + L.mslot);
-  v.cells.overflow = (int32_t*)(base + L.covf);
+  v.cells.count = gmem((int32_t*)(base + L.ccount));
+  v.cells.slots = gmem((int32_t*)(base + L.cslots));
+  v.cells.mol_cell = gmem((int32_t*)(base + L.mcell));
+  v.cells.mol_slot = gmem((int32_t*)(base + L.mslot));
+  v.cells.overflow = gmem((int32_t*)(base + L.covf));
   v.cells.cap = L.cap;
-  v.verlet.stale = (int32_t*)(base + L.vstale);
-  v.verlet.count = (int32_t*)(base + L.vcount);
+  v.verlet.stale = gmem((int32_t*)(base + L.vstale));
+  v.verlet.count = gmem((int32_t*)(base + L.vcount));
   E.rng = b->d_rng + s;
   E.max_t = d->max_translate;
   E.max_r = d->max_rotate;

@@ -499,7 +499,7 @@ int bin_cells(mcgpu_box* b, double reach) {
     MCGPU_HIP_TRY(hipMalloc(&b->cell_count, sizeof(int32_t) * ncell));
     b->ncell_alloc = ncell;
   }
-  g.count = b->cell_count;
+  g.count = gmem(b->cell_count);
   int cap = std::max(8, (int)std::ceil(2.0 * b->bv.n_mol / ncell) + 8);
   cap = (cap + 3) & ~3;
   for (int attempt = 0; attempt < 8; attempt++) {
@@ -510,7 +510,7 @@ int bin_cells(mcgpu_box* b, double reach) {
       b->slots_alloc = ncell * cap;
     }
     g.cap = cap;
-    g.slots = b->cell_slots;
+    g.slots = gmem(b->cell_slots);
     hipLaunchKernelGGL(k_cells_clear, dim3((ncell + 255) / 256), dim3(256), 0,
                        b->stream, g, ncell);
     hipLaunchKernelGGL(k_cells_assign, dim3((b->bv.n_mol + 255) / 256), dim3(256),
@@ -763,22 +763,22 @@ extern "C" int mcgpu_box_create(const mcgpu_box_desc* d, int precision,
   ALLOC(b->d_partial, nm);
   ALLOC(b->d_pairs, nm);
 #undef ALLOC
-  v.x = x;
-  v.y = y;
-  v.z = z;
-  v.px = px;
-  v.py = py;
-  v.pz = pz;
-  v.mol_start = ms;
-  v.mol_len = ml;
-  v.mol_pidx_start = mps;
-  v.mol_pidx_count = mpc;
-  v.mol_param = mp;
-  v.meta = mm;
+  v.x = gmem(x);
+  v.y = gmem(y);
+  v.z = gmem(z);
+  v.px = gmem(px);
+  v.py = gmem(py);
+  v.pz = gmem(pz);
+  v.mol_start = gmem(ms);
+  v.mol_len = gmem(ml);
+  v.mol_pidx_start = gmem(mps);
+  v.mol_pidx_count = gmem(mpc);
+  v.mol_param = gmem(mp);
+  v.meta = gmem(mm);
   v.single_primary = pt.single_primary;
-  v.pidx = pi;
-  v.p64 = p64;
-  v.p32 = p32;
+  v.pidx = gmem(pi);
+  v.p64 = gmem(p64);
+  v.p32 = gmem(p32);
   auto fail = [&](int code) {
     free_box(b);
     return code;

@@ -48,24 +48,41 @@ struct alignas(16) MolMeta {
   int32_t pstart;
 };
 
+// Box arrays are global memory.  The batch kernel keeps its box descriptor
+// in LDS, and a pointer read back from LDS has no known address space: the
+// compiler would reach the arrays with flat instructions, which also count
+// against the LDS wait counter (an LDS read after one waits for it).  Device
+// code therefore declares the descriptor's pointers global.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MCGPU_GMEM __attribute__((address_space(1)))
+#else
+#define MCGPU_GMEM
+#endif
+
+// Host code fills the descriptors from plain device pointers.
+template <class T>
+inline MCGPU_GMEM T* gmem(T* p) {
+  return (MCGPU_GMEM T*)p;
+}
+
 struct CellGrid {
   int32_t n[3];       // cells per dimension (>= 1)
   int32_t cap;        // slots per cell
   double inv_w[3];    // 1 / cell width
-  int32_t* count;     // [ncell]
-  int32_t* slots;     // [ncell * cap]
-  int32_t* mol_cell;  // [n_mol]
-  int32_t* mol_slot;  // [n_mol]
-  int32_t* overflow;  // [1]: nonzero -> lists incomplete, scan all molecules
+  MCGPU_GMEM int32_t* count;     // [ncell]
+  MCGPU_GMEM int32_t* slots;     // [ncell * cap]
+  MCGPU_GMEM int32_t* mol_cell;  // [n_mol]
+  MCGPU_GMEM int32_t* mol_slot;  // [n_mol]
+  MCGPU_GMEM int32_t* overflow;  // [1]: nonzero -> lists incomplete, scan all molecules
 };
 
 struct VerletList {
   int32_t cap;        // entries per molecule
   double skin_half2;  // (skin/2)^2
-  int32_t* count;     // [n_mol]
-  int32_t* list;      // [n_mol * cap]
-  double* ref;        // [3*n_mol] first-primary position at build
-  int32_t* stale;     // [1]: nonzero -> some molecule moved > skin/2
+  MCGPU_GMEM int32_t* count;     // [n_mol]
+  MCGPU_GMEM int32_t* list;      // [n_mol * cap]
+  MCGPU_GMEM double* ref;        // [3*n_mol] first-primary position at build
+  MCGPU_GMEM int32_t* stale;     // [1]: nonzero -> some molecule moved > skin/2
 };
 
 #define MCGPU_NB_BRUTE 0
@@ -78,21 +95,21 @@ struct VerletList {
 #define MCGPU_FAULT_MOLLEN 2      // molecule length outside [1, kMaxSites]
 
 struct BoxView {
-  double* x;
-  double* y;
-  double* z;
-  double* px;
-  double* py;
-  double* pz;
-  const int32_t* mol_start;
-  const int32_t* mol_len;
-  const int32_t* mol_pidx_start;
-  const int32_t* mol_pidx_count;
-  const int32_t* mol_param;
-  const MolMeta* meta;  // [n_mol]
-  const int32_t* pidx;
-  const SiteParam64* p64;
-  const SiteParam32* p32;
+  MCGPU_GMEM double* x;
+  MCGPU_GMEM double* y;
+  MCGPU_GMEM double* z;
+  MCGPU_GMEM double* px;
+  MCGPU_GMEM double* py;
+  MCGPU_GMEM double* pz;
+  const MCGPU_GMEM int32_t* mol_start;
+  const MCGPU_GMEM int32_t* mol_len;
+  const MCGPU_GMEM int32_t* mol_pidx_start;
+  const MCGPU_GMEM int32_t* mol_pidx_count;
+  const MCGPU_GMEM int32_t* mol_param;
+  const MCGPU_GMEM MolMeta* meta;  // [n_mol]
+  const MCGPU_GMEM int32_t* pidx;
+  const MCGPU_GMEM SiteParam64* p64;
+  const MCGPU_GMEM SiteParam32* p32;
   int32_t n_mol;
   int32_t n_site;
   int32_t max_len;
@@ -299,11 +316,15 @@ struct CellRange {
   int lo[3];
   int cnt[3];
   __device__ int total() const { return cnt[0] * cnt[1] * cnt[2]; }
+  // k < 64 (callers hold at most 64 cells): the quotients by the small
+  // counts come from the hardware reciprocal (v_rcp_f32, relative error
+  // about 2^-23), (k + 0.5) / m being at least 0.5 / m from an integer, far
+  // beyond that error
   __device__ int cell(const CellGrid& g, int k) const {
-    const int c2 = k % cnt[2];
-    const int t = k / cnt[2];
-    const int c1 = t % cnt[1];
-    const int c0 = t / cnt[1];
+    const int t = (int)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)cnt[2]));
+    const int c2 = k - t * cnt[2];
+    const int c0 = (int)(((float)t + 0.5f) * __builtin_amdgcn_rcpf((float)cnt[1]));
+    const int c1 = t - c0 * cnt[1];
     int a = lo[0] + c0, b = lo[1] + c1, c = lo[2] + c2;
     a += (a < 0) ? g.n[0] : 0;
     a -= (a >= g.n[0]) ? g.n[0] : 0;

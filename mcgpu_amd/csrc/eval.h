@@ -30,6 +30,7 @@ template <int NT>
 struct EvalShared {
   static constexpr int kCap = NT * 2 < 256 ? 256 : NT * 2;
   static constexpr int kItemCap = 2048;  // items with a cell entry in item_cell
+  static constexpr int kCellLds = 1024;  // cells whose counts a launch keeps in LDS
   // staged hit sites; 1024-lane groups keep their LDS under 64 KiB, which
   // LDS instructions address from one base with an immediate offset
   static constexpr int kSiteCap = NT >= 1024 ? 768 : (NT >= 512 ? 1536 : (NT >= 256 ? 768 : 192));
@@ -61,6 +62,9 @@ struct EvalShared {
   int32_t cell_id[64];       // cells of the current range (at most 4x4x4)
   int32_t cell_pre[65];      // prefix of their occupancies
   uint8_t item_cell[kItemCap];  // range cell of each candidate item
+  // copy of the box's cell occupancies, kept by launches that hold the box
+  // for many moves (the batch kernel); prepare_cells_wave reads it when asked
+  int32_t ccount[kCellLds];
   double red[kNW];
   double red2[kNW];
   unsigned long long redc[kNW];  // sites << 32 | candidates, per wave
@@ -154,7 +158,7 @@ __device__ __forceinline__ int div_small(int u, unsigned long long magic) {
 template <int NT, bool MIXED>
 __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh,
                                            int nhits, double& e_old, double& e_new,
-                                           unsigned& npairs, unsigned& nsites) {
+                                           unsigned& npairs) {
   constexpr int CAP = EvalShared<NT>::kSiteCap;
   const int M = bv.max_len;
   const unsigned long long magic = (1ull << 32) / (unsigned)M + 1ull;
@@ -210,7 +214,6 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
           yb[v] = sh.sy[i];
           zb[v] = sh.sz[i];
         }
-        nsites += (pi[v] >= 0 && a == 0);
         if (!lane_real) pi[v] = -1;
       }
 #pragma unroll
@@ -276,14 +279,17 @@ __device__ __forceinline__ int pick_source(const BoxView& bv, const double* nx,
 }
 
 // Wave 0 (all 64 lanes): the cells around old cell co and proposed cell cn,
-// their occupancies, the prefix and the item -> cell map.  Returns the cell
-// count, or -1 when the range exceeds 64 cells (the caller then scans every
-// molecule); *nitems receives the candidate count.
+// their occupancies (from sh.ccount when lds_counts), the prefix and, when
+// fill, the item -> cell map (else the caller fills it with
+// fill_item_cells).  Returns the cell count, or -1 when the range exceeds 64
+// cells (the caller then scans every molecule); *nitems receives the
+// candidate count.
 template <int NT>
 __device__ __forceinline__ int prepare_cells_wave(const BoxView& bv,
                                                   EvalShared<NT>& sh,
                                                   const int co[3], const int cn[3],
-                                                  int* nitems) {
+                                                  int* nitems, bool lds_counts = false,
+                                                  bool fill = true) {
   const CellGrid& g = bv.cells;
   const CellRange range = make_range(g, co, cn);
   const int ncells = range.total();
@@ -292,17 +298,35 @@ __device__ __forceinline__ int prepare_cells_wave(const BoxView& bv,
   int n = 0, c = 0;
   if (lane < ncells) {
     c = range.cell(g, lane);
-    n = g.count[c];
+    n = lds_counts ? sh.ccount[c] : g.count[c];
     n = n < g.cap ? n : g.cap;
     sh.cell_id[lane] = c;
   }
   const int incl = wave_incl_scan_i32(n);
   sh.cell_pre[lane + 1] = incl;
   if (lane == 0) sh.cell_pre[0] = 0;
-  const int end = incl < EvalShared<NT>::kItemCap ? incl : EvalShared<NT>::kItemCap;
-  for (int k = incl - n; k < end; k++) sh.item_cell[k] = (uint8_t)lane;
+  if (fill) {
+    const int end = incl < EvalShared<NT>::kItemCap ? incl : EvalShared<NT>::kItemCap;
+    for (int k = incl - n; k < end; k++) sh.item_cell[k] = (uint8_t)lane;
+  }
   *nitems = readlane_i32(incl, 63);
   return ncells;
+}
+
+// All NT threads: the item -> cell map of the ncells cells whose prefix
+// prepare_cells_wave left in sh.cell_pre, NT / 64 threads per cell.  The
+// caller places a barrier before and after it.
+template <int NT>
+__device__ __forceinline__ void fill_item_cells(EvalShared<NT>& sh, int ncells) {
+  constexpr int kPer = NT / 64;
+  const int c = threadIdx.x / kPer;
+  if (c < ncells) {
+    const int lim = EvalShared<NT>::kItemCap;
+    int end = sh.cell_pre[c + 1];
+    end = end < lim ? end : lim;
+    for (int k = sh.cell_pre[c] + (int)threadIdx.x % kPer; k < end; k += kPer)
+      sh.item_cell[k] = (uint8_t)c;
+  }
 }
 
 // The range cell holding candidate `item` of the cell source.
@@ -330,17 +354,22 @@ struct EvalWork {
 // ncells and the cell tables), for molecule `mol` at the LDS-held old
 // position and, when has_new, the proposed one.  Called by all NT threads
 // after a barrier that follows the preparation.  Leaves per-wave partial
-// sums in sh.red / red2 / redc / redp and ends with a barrier.
+// sums in sh.red / red2 / redc / redp and ends with a barrier.  The
+// candidate and site counts are per-wave ballot counts; the site-pair count
+// (redp) is summed only when count_pairs.
 template <int NT, bool MIXED>
 __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
-                                          int mol, int jlo, bool has_new) {
+                                          int mol, int jlo, bool has_new,
+                                          bool count_pairs = true) {
   const int t = threadIdx.x;
   const int src = __builtin_amdgcn_readfirstlane(sh.src);
   const int nitems = __builtin_amdgcn_readfirstlane(sh.nitems);
   const int ncells = __builtin_amdgcn_readfirstlane(sh.ncells);
   const int npri = __builtin_amdgcn_readfirstlane(sh.npri);
+  const int uni_len = bv.uni_len;
   double eo = 0.0, en = 0.0;
-  unsigned ncand = 0, nsites = 0, npairs = 0;
+  unsigned npairs = 0;
+  unsigned ncand_w = 0, nsites_w = 0;  // this wave's candidates and hit sites
   int nhits = 0, par = 0;
   for (int base = 0; base < nitems; base += NT) {
     const int item = base + t;
@@ -357,9 +386,17 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
     }
     int f = 0;
     MolMeta mj{0, 0, 0, 0, 0};
-    if (j >= jlo && j != mol) {
-      ncand++;
-      mj = bv.meta[j];
+    const bool cand = j >= jlo && j != mol;
+    ncand_w += (unsigned)__popcll(__ballot(cand));
+    if (cand) {
+      if (uni_len > 0) {  // uniform box: the record follows from the index
+        mj.start = j * uni_len;
+        mj.len = (int16_t)uni_len;
+        mj.param = bv.uni_param;
+        mj.pstart = j;
+      } else {
+        mj = bv.meta[j];
+      }
       int ps = j, pc = 1;  // one primary per molecule: no indirection
       if (!bv.single_primary) {
         ps = mj.pstart;
@@ -381,6 +418,10 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
     }
     // compaction: rank within the wave, wave counts through LDS
     const unsigned long long bal = __ballot(f != 0);
+    if (uni_len > 0)
+      nsites_w += (unsigned)__popcll(bal) * (unsigned)uni_len;
+    else
+      nsites_w += wave_sum_u32(f ? (unsigned)mj.len : 0u);
     if (wlane() == 0) sh.scan[par][wave_id()] = __popcll(bal);
     __syncthreads();
     int pre = 0, total = 0;
@@ -393,7 +434,7 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
     par ^= 1;
     if (nhits + total > EvalShared<NT>::kCap) {  // block-uniform
       // (ends with a barrier: every lane is done with the hits before reuse)
-      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs, nsites);
+      pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs);
       nhits = 0;
     }
     if (f) {
@@ -405,16 +446,14 @@ __device__ __forceinline__ void eval_core(const BoxView& bv, EvalShared<NT>& sh,
     nhits += total;
   }
   __syncthreads();
-  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs, nsites);
+  if (nhits > 0) pair_phase<NT, MIXED>(bv, sh, nhits, eo, en, npairs);
   eo = wave_sum_f64(eo);
   en = wave_sum_f64(en);
-  const unsigned wcand = wave_sum_u32(ncand);
-  const unsigned wsites = wave_sum_u32(nsites);
-  const unsigned wpairs = wave_sum_u32(npairs);
+  const unsigned wpairs = count_pairs ? wave_sum_u32(npairs) : 0u;
   if (wlane() == 0) {
     sh.red[wave_id()] = eo;
     sh.red2[wave_id()] = en;
-    sh.redc[wave_id()] = ((unsigned long long)wsites << 32) | wcand;
+    sh.redc[wave_id()] = ((unsigned long long)nsites_w << 32) | ncand_w;
     sh.redp[wave_id()] = wpairs;
   }
   __syncthreads();
@@ -507,8 +546,9 @@ __device__ __forceinline__ void eval_molecule(const BoxView& bv, int mol, int jl
 
 // Moves molecule `mol` from cell cold to cell cnew in the cell lists (one
 // lane): swap-remove from the old cell, append to the new one; a full cell
-// marks the lists incomplete (scans then fall back to every molecule).
-__device__ __forceinline__ void cell_move(const CellGrid& g, int mol, int cold,
+// marks the lists incomplete (scans then fall back to every molecule), and
+// the call returns false.
+__device__ __forceinline__ bool cell_move(const CellGrid& g, int mol, int cold,
                                           int cnew) {
   const int slot = g.mol_slot[mol];
   const int last = g.count[cold] - 1;
@@ -522,9 +562,10 @@ __device__ __forceinline__ void cell_move(const CellGrid& g, int mol, int cold,
     g.mol_slot[mol] = n;
     g.count[cnew] = n + 1;
     g.mol_cell[mol] = cnew;
-  } else {
-    g.overflow[0] = 1;
+    return true;
   }
+  g.overflow[0] = 1;
+  return false;
 }
 
 // Writes the LDS-held proposed coordinates of `mol` into the box and keeps
