@@ -99,8 +99,7 @@ struct VolItem {
 template <int NT>
 struct BatchShared {
   EvalShared<NT> ev;
-  BatchBox tb;   // this box's descriptor, copied once per launch
-  BoxCtl c;      // its control block (wave 0, lane 0 updates it)
+  BoxCtl c;      // the box's control block (wave 0, lane 0 updates it)
   int32_t go;    // the current move runs (else the box stops this launch)
   int32_t mol, start, pstart;
   int32_t cells_live;  // cell lists in use and complete for this move
@@ -124,10 +123,10 @@ struct BatchShared {
 // 387-422, 545-565, 710-729).  FMA contraction is off so the doubles match
 // the reference's.
 template <int NT>
-__device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s,
+__device__ __forceinline__ void front_move(BatchShared<NT>& S, const BatchBox& B, WaveRng& g, int s,
                                            long long budget, long long t_start) {
 #pragma clang fp contract(off)
-  const BoxView& bv = S.tb.bv;
+  const BoxView& bv = B.bv;
   EvalShared<NT>& sh = S.ev;
   const int lane = wlane();
   bool go = S.c.steps_done < S.c.next_stop;
@@ -148,11 +147,11 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
   if (lane == 0) {
     b = (double)n;
   } else if (lane >= 2 && lane < 5) {
-    a = -S.tb.max_t;
-    b = S.tb.max_t;
+    a = -B.max_t;
+    b = B.max_t;
   } else if (lane >= 5 && lane < 8) {
-    a = -S.tb.max_r;
-    b = S.tb.max_r;
+    a = -B.max_r;
+    b = B.max_r;
   }
   const double U = (b - a) * q + a;
   int mol = (int)readlane_f64(U, 0);
@@ -300,8 +299,8 @@ __device__ __forceinline__ void front_move(BatchShared<NT>& S, WaveRng& g, int s
 // trace, and on acceptance the commit of the proposed sites, primary mirror
 // and cell lists.
 template <int NT>
-__device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
-  const BoxView& bv = S.tb.bv;
+__device__ __forceinline__ void finish_move(BatchShared<NT>& S, const BatchBox& B, WaveRng& g) {
+  const BoxView& bv = B.bv;
   const EvalShared<NT>& sh = S.ev;
   const int lane = wlane();
   double eo, en;
@@ -321,10 +320,10 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
     S.c.work_cand += (unsigned)(wc & 0xffffffffu);
     S.c.work_sites += (unsigned)(wc >> 32);
     S.c.work_self += 2 * len;
-    if (S.tb.record) {
-      S.tb.tr_mol[step] = mol;
-      S.tb.tr_de[step] = de;
-      S.tb.tr_acc[step] = (int8_t)acc;
+    if (B.record) {
+      B.tr_mol[step] = mol;
+      B.tr_de[step] = de;
+      B.tr_acc[step] = (int8_t)acc;
     }
     if (acc) {
       S.c.energy += de;
@@ -369,33 +368,32 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, WaveRng& g) {
 // since the work-group started (0 = no time limit).  Wave 0 draws, builds,
 // decides and commits each move; all waves scan candidates and sum pairs.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
+__global__ __launch_bounds__(NT) void k_batch(const BatchBox* __restrict__ table,
                                               BoxCtl* ctl,
                                               const int32_t* active,
                                               int sweeps, long long budget) {
   __shared__ BatchShared<NT> S;
-  constexpr int kWords = (int)(sizeof(BatchBox) / 4);
-  static_assert(sizeof(BatchBox) % 4 == 0 && kWords <= NT, "descriptor copy");
   const int t = threadIdx.x;
   const bool w0 = t < 64;
   // active = [count, slot 0, slot 1, ...]; the grid is rounded up
   if ((int)blockIdx.x >= __builtin_amdgcn_readfirstlane(active[0])) return;
   const int slot = __builtin_amdgcn_readfirstlane(active[1 + blockIdx.x]);
   const long long t_start = wall_clock64();
-  if (t < kWords)
-    reinterpret_cast<int32_t*>(&S.tb)[t] = reinterpret_cast<const int32_t*>(table + slot)[t];
+  // the descriptor is read in place (scalar loads from a table this launch
+  // does not write)
+  const BatchBox& B = table[slot];
   if (t == 0) S.c = ctl[slot];
   __syncthreads();
   WaveRng g{0, 0, 0};
   if (w0) {
-    const Rng31* rs = S.tb.rng;
+    const Rng31* rs = B.rng;
     const int lane = wlane();
     g.s = lane < 31 ? rs->s[lane] : 0;
     g.f = __builtin_amdgcn_readfirstlane(rs->f);
     g.r = __builtin_amdgcn_readfirstlane(rs->r);
   }
-  const BoxView& bv = S.tb.bv;
-  const bool mixed = S.tb.precision == MCGPU_MIXED;
+  const BoxView& bv = B.bv;
+  const bool mixed = B.precision == MCGPU_MIXED;
   load_params<NT>(bv, S.ev);
   // the cell occupancies into LDS, when the grid fits: only this launch's
   // commits change them until it ends
@@ -410,7 +408,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
   }
   __syncthreads();
   for (int s = 0; s < sweeps; s++) {
-    if (w0) front_move<NT>(S, g, s, budget, t_start);
+    if (w0) front_move<NT>(S, B, g, s, budget, t_start);
     __syncthreads();
     if (!S.go || S.ev.fault) break;
     if (__builtin_amdgcn_readfirstlane(S.ev.src) == MCGPU_NB_CELL) {
@@ -422,11 +420,11 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* table,
       eval_core<NT, true>(bv, S.ev, mol, 0, true, false);
     else
       eval_core<NT, false>(bv, S.ev, mol, 0, true, false);
-    if (w0) finish_move<NT>(S, g);
+    if (w0) finish_move<NT>(S, B, g);
   }
   if (w0) {
     const int lane = wlane();
-    Rng31* rs = S.tb.rng;
+    Rng31* rs = B.rng;
     if (lane < 31) rs->s[lane] = g.s;
     if (lane == 0) {
       rs->f = g.f;
