@@ -48,11 +48,12 @@ struct alignas(16) MolMeta {
   int32_t pstart;
 };
 
-// Box arrays are global memory.  The batch kernel keeps its box descriptor
-// in LDS, and a pointer read back from LDS has no known address space: the
-// compiler would reach the arrays with flat instructions, which also count
-// against the LDS wait counter (an LDS read after one waits for it).  Device
-// code therefore declares the descriptor's pointers global.
+// Box arrays are global memory.  Kernels read their box descriptors from
+// memory (the batch table), and a pointer read back from memory has no
+// known address space: the compiler would reach the arrays with flat
+// instructions, which also count against the LDS wait counter (an LDS read
+// after one waits for it).  Device code therefore declares the descriptor's
+// pointers global.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define MCGPU_GMEM __attribute__((address_space(1)))
 #else
