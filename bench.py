@@ -410,7 +410,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20,
                     help=f"timed steps; a step is {JOBS_PER_STEP} jobs of the stream")
-    ap.add_argument("--warmup", type=int, default=2,
+    ap.add_argument("--warmup", type=int, default=5,
                     help=f"untimed warm-up steps ({JOBS_PER_STEP} jobs each, "
                     "at most 2000 moves per job, all arriving at once)")
     ap.add_argument("--sweeps", type=int, default=2000, help="max moves per box per tick")
