@@ -389,17 +389,21 @@ def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
     return out
 
 
-def pmc_traffic(steps: int, warmup: int):
-    """HBM bytes per launch of the batch kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled), used only
-    when that profile was taken on this same workload (the same --steps and
-    --warmup); None otherwise."""
+def pmc_traffic_ratio(steps: int, warmup: int):
+    """HBM bytes per algorithmic byte of the batch kernel, from the committed
+    rocprofv3 counter summary of this same workload (the same --steps and
+    --warmup): FETCH_SIZE + WRITE_SIZE of every k_batch dispatch of the
+    counted runs, scaled by a copy kernel's readings, over the algorithmic
+    bytes those runs' serving loops reported.  None when the profile was
+    taken on another workload.  (Counters slow the kernels, so ticks of a
+    counted run carry more moves than ticks of this run; the ratio, not the
+    bytes per launch, carries over.)"""
     p = ROOT / "profiles" / "k_batch_serving_pmc.txt"
     if not p.exists():
         return None
     text = p.read_text()
     args = re.search(r"bench_args:\s*--steps (\d+) --warmup (\d+)", text)
-    val = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    val = re.search(r"traffic_per_algorithmic_byte:\s*([0-9.eE+]+)", text)
     if not args or not val or (int(args.group(1)), int(args.group(2))) != (steps, warmup):
         return None
     return float(val.group(1))
@@ -496,7 +500,12 @@ def main() -> int:
     alg_bytes = 24.0 * (st.work_candidates + st.work_sites)
     achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
     per_launch = alg_bytes / max(st.ticks, 1)
-    traffic = pmc_traffic(args.steps, args.warmup)
+    ratio = pmc_traffic_ratio(args.steps, args.warmup)
+    traffic = ratio * per_launch if ratio is not None else None
+    # every serving loop of the process (warm-up, timed, capacity): what a
+    # counter profile of this command covers
+    alg_bytes_run = 24.0 * sum(float(x.work_candidates + x.work_sites)
+                               for x in ([wst] if warm else []) + [st, cst])
 
     plain_rate, plain_jobs, plain_moves = plain_sample(timed, pool, args.plain_seconds)
     print(f"# plain FP64 path: {plain_rate:.4g} moves/s over {plain_jobs} jobs",
@@ -555,12 +564,15 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_per_algorithmic_byte": ratio,
             "kernel": "k_batch (one graph launch: 256-, 512- and 1024-lane instances on three streams)",
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": st.kernel_ms_total / max(st.ticks, 1),
-            # graph launches of the whole process (warm-up, timed and
-            # capacity serving), to normalise per-process counter totals
+            # graph launches and algorithmic bytes of the whole process
+            # (warm-up, timed and capacity serving), to normalise
+            # per-process counter totals
             "graph_launches_in_run": int(wticks + st.ticks + cst.ticks),
+            "algorithmic_bytes_in_run": alg_bytes_run,
         },
         "full_energy": full,
         "fp32_pair_de_error_kcal_mol": prec,
