@@ -30,12 +30,14 @@ Beside it:
   - the full-energy kernel's site-pair rate against its VALU roof, and the
     per-move dE error of FP32 pair terms against FP64 on a box of each
     configuration;
+  - configurations 0, 1 and 3 one box at a time: plain FP64 against the
+    batched loop, and the move kernel's algorithmic GB/s (key "per_config");
   - 512 water boxes of 64-4,096 molecules advanced together for 20,000
     moves each (key "batch512").
 Separate modes print one JSON object each and skip the stream:
---full-energy-only, --precision-only, and --batch-512 (512 water boxes of
-64-4,096 molecules advanced together for 20,000 moves each, against the
-plain path).
+--full-energy-only, --precision-only, --per-config-only, and --batch-512
+(512 water boxes of 64-4,096 molecules advanced together for 20,000 moves
+each, against the plain path).
 """
 from __future__ import annotations
 
@@ -372,9 +374,14 @@ def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
         while not all(b.stats(s).done for s in slots):
             b.run(sweeps)
         dt = time.perf_counter() - t0
-        moves = sum(b.stats(s).steps_done for s in slots)
+        st = [b.stats(s) for s in slots]
+        moves = sum(x.steps_done for x in st)
         out[f"{name}_moves_per_s"] = moves / dt
         out[f"{name}_seconds"] = dt
+        # move kernel: algorithmic bytes (24 B per candidate primary and per
+        # site loaded) over the pass's wall time
+        out[f"{name}_algorithmic_gb_per_s"] = 24.0 * sum(
+            x.work_candidates + x.work_sites for x in st) / dt / 1e9
         b.close()
     sample = boxes[::32]
     moves = t = 0.0
@@ -386,6 +393,45 @@ def batch512_measure(n_boxes: int = 512, sweeps: int = 20000, seed: int = 512):
     out["plain_fp64_moves_per_s"] = moves / t
     out["plain_sample"] = f"{len(sample)} of the boxes, 2000 moves each"
     out["ratio_mixed_vs_plain"] = out["mixed_moves_per_s"] / out["plain_fp64_moves_per_s"]
+    return out
+
+
+def per_config_measure(moves: int = 4000):
+    """Configurations 0, 1 and 3 one box at a time (NVT; the NPT volume
+    moves of configuration 3 are measured by full_energy): the plain FP64
+    path (host RNG, one fused trial-move launch per move, PCIe included) and
+    the batched device loop on the same box and seed, with FP64 and with
+    FP32 pair terms.  For the batched loop: moves per second of kernel time
+    (one box holds one CU, so this is the per-move latency) and the move
+    kernel's algorithmic GB/s (24 B per candidate primary and per site
+    loaded) over that kernel time."""
+    import mcgpu_amd as m
+    out = {}
+    for tag, name in (("c0_water256", "water256"), ("c1_bicyclic4000", "bicyclic4000"),
+                      ("c3_mixture20000", "mixture20000")):
+        hb = m.workload(name)
+        nb = neighbor_for(hb)
+        res, _ = m.plain_run(hb, moves, precision=m.FP64, neighbor=nb, trace=False)
+        row = {"moves": moves,
+               "neighbor": "cells" if nb == m.NEIGHBOR_CELL else "all pairs",
+               "plain_fp64_moves_per_s": moves / res.loop_seconds}
+        for prec, p in ((m.FP64, "fp64"), (m.MIXED, "mixed")):
+            b = m.Batch(2)
+            b.set_tick_budget(2000.0)
+            s = b.add(hb, steps=moves, precision=prec, neighbor=nb)
+            while not b.stats(s).done:
+                b.run(2000)
+            avg_ms, launches = b.kernel_ms()  # since the batch was made
+            ms = avg_ms * launches
+            st = b.stats(s)
+            row[f"batched_{p}_moves_per_s"] = st.steps_done / (ms / 1e3)
+            row[f"batched_{p}_us_per_move"] = 1e3 * ms / st.steps_done
+            row[f"batched_{p}_algorithmic_gb_per_s"] = (
+                24.0 * (st.work_candidates + st.work_sites) / (ms / 1e3) / 1e9)
+            b.close()
+        row["ratio_batched_mixed_vs_plain"] = (row["batched_mixed_moves_per_s"]
+                                               / row["plain_fp64_moves_per_s"])
+        out[tag] = row
     return out
 
 
@@ -431,6 +477,9 @@ def main() -> int:
                     help="measure only the FP32-pair dE error and print its JSON")
     ap.add_argument("--batch-512", action="store_true",
                     help="measure only 512 water boxes advanced together and print its JSON")
+    ap.add_argument("--per-config-only", action="store_true",
+                    help="measure only configurations 0, 1 and 3 one box at a time "
+                    "(plain vs batched, move-kernel GB/s) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -449,6 +498,9 @@ def main() -> int:
         return 0
     if args.batch_512:
         print(json.dumps(batch512_measure()), flush=True)
+        return 0
+    if args.per_config_only:
+        print(json.dumps(per_config_measure()), flush=True)
         return 0
 
     t_setup = time.perf_counter()
@@ -515,6 +567,8 @@ def main() -> int:
     print(f"# full energy: {json.dumps(full)}", file=sys.stderr, flush=True)
     prec = precision_error_measure()
     print(f"# FP32-pair dE error: {json.dumps(prec)}", file=sys.stderr, flush=True)
+    per_config = per_config_measure()
+    print(f"# per configuration: {json.dumps(per_config)}", file=sys.stderr, flush=True)
 
     b512 = None
     if not args.no_batch512:
@@ -576,6 +630,7 @@ def main() -> int:
         },
         "full_energy": full,
         "fp32_pair_de_error_kcal_mol": prec,
+        "per_config": per_config,
         "batch512": b512,
         "cpu_baseline": cpu,
     }
