@@ -152,6 +152,31 @@ def test_batch_cells_equal_brute_over_boundary_crossings():
     assert de_ok(tc.de, tb.de)
 
 
+def test_batch_cells_beyond_the_lds_occupancy_copy():
+    """A grid of 11^3 = 1,331 cells (water, rc 5 A): more cells than a launch
+    keeps occupancies for in LDS (1,024), so the moves read and update the
+    occupancies in global memory; the trace must equal the all-molecule
+    trace, with no fallback to the all-molecule scan.  A 4 x 4 x 4 grid of
+    the same kind takes the LDS copy, checked the same way."""
+    for n, want_cells in ((6000, 11 ** 3), (400, 4 ** 3)):
+        hb = m.HostBox.generate(m.WATER, n, density=1.0, cutoff=5.0, seed=92 + n,
+                                mc_seed=5)
+        d = hb.desc()
+        assert np.prod([int(np.floor(L / 5.0)) for L in d.box]) == want_cells
+        steps = 6000
+        b = m.Batch(4)
+        s_cell = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_CELL, record_trace=True)
+        s_brute = b.add(hb, steps=steps, neighbor=m.NEIGHBOR_BRUTE, record_trace=True)
+        st = run_all(b, [s_cell, s_brute], steps, 2000)
+        assert all(x.fault == 0 for x in st)
+        assert st[0].neighbor_fallbacks == 0
+        tc, tb = b.trace(s_cell), b.trace(s_brute)
+        assert np.array_equal(tc.mol, tb.mol)
+        assert np.array_equal(tc.accepted, tb.accepted)
+        assert de_ok(tc.de, tb.de)
+        b.close()
+
+
 def test_batch_autotuner_picks_a_size_and_keeps_results():
     """With a tick budget, the first box of a class runs one full tick at each
     work-group size and the class keeps the size with the most moves per tick.
