@@ -17,16 +17,22 @@ def de_ok(got, want, scale=None):
     may differ from glibc's in the last bit.  The relative term matters only
     at trial positions overlapping another molecule (energies up to 1e12
     kcal/mol in the generated 16-site boxes); on the 256-water box every
-    move meets the absolute 1e-9 bound."""
+    move meets the absolute 1e-9 bound.  Moves of physical size (energy
+    scale below 1e3 kcal/mol) must also meet 2e-10 absolutely, whatever the
+    box (the largest error measured over every GPU test's moves of that
+    size is 3.8e-11)."""
     if isinstance(want, (dict, np.lib.npyio.NpzFile)):
         scale = np.maximum(np.abs(want["old_e"]), np.abs(want["new_e"]))
         want = want["de"]
     elif scale is None:
         scale = np.abs(want)
     err = np.abs(got - want)
+    phys = scale < 1e3
+    phys_max = err[phys].max() if phys.any() else 0.0
     print(f"max |dE err| {err.max():.3e}, max err/scale "
-          f"{np.max(err / np.maximum(scale, 1e-300)):.3e}")
-    return np.all(err <= DE_TOL + 1e-12 * scale)
+          f"{np.max(err / np.maximum(scale, 1e-300)):.3e}, max on moves of "
+          f"physical size {phys_max:.3e}")
+    return bool(np.all(err <= DE_TOL + 1e-12 * scale) and phys_max <= 2e-10)
 
 
 def run_all(batch, slots, total, chunk):
