@@ -18,6 +18,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <deque>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -52,7 +53,9 @@ struct mcgpu_server {
   std::mutex mu;                 // guards everything below
   std::condition_variable cv_loop;  // submissions and stop wake the loop
   std::condition_variable cv_done;  // completions wake waiters
-  std::vector<Job> jobs;
+  // a deque: a submission (push_back) leaves references to earlier jobs
+  // valid, which the loop holds across its unlocked device calls
+  std::deque<Job> jobs;
   std::vector<int64_t> pending;  // queued job ids, in arrival order
   mcgpu_server_stats stats{};
   int64_t completed = 0;
@@ -413,9 +416,9 @@ extern "C" int mcgpu_server_poll(mcgpu_server* s, int64_t job_id, mcgpu_job_resu
 
 extern "C" int mcgpu_server_results(mcgpu_server* s, mcgpu_job_result* out,
                                     int64_t cap, int64_t* n) {
-  if (!s || !n) return MCGPU_ERR_INVALID;
+  if (!s || !n || (cap > 0 && !out)) return MCGPU_ERR_INVALID;
   std::lock_guard<std::mutex> lock(s->mu);
-  const int64_t k = std::min<int64_t>(cap, (int64_t)s->jobs.size());
+  const int64_t k = std::max<int64_t>(0, std::min<int64_t>(cap, (int64_t)s->jobs.size()));
   for (int64_t i = 0; i < k; i++) out[i] = s->jobs[i].res;
   *n = k;
   return MCGPU_OK;
