@@ -242,6 +242,9 @@ int mcgpu_box_footprint(const mcgpu_box_desc* desc,
  * MCGPU_ERR_NOMEM if the budget or slot count is exhausted. */
 int mcgpu_batch_add(mcgpu_batch* b, const mcgpu_box_desc* desc,
                     const mcgpu_run_params* params, int32_t* slot);
+/* Frees a slot.  It waits for a tick in flight only when that tick runs the
+ * slot's box (a box that was done or paused when the tick was launched is
+ * freed at once). */
 int mcgpu_batch_remove(mcgpu_batch* b, int32_t slot);
 
 /* Everything a box needs to continue its move sequence exactly where it
@@ -283,9 +286,18 @@ int mcgpu_batch_add_resumed(mcgpu_batch* b, const mcgpu_box_desc* desc,
                             const mcgpu_resume_state* r, int32_t* slot);
 /* Advances every active box by up to `sweeps` moves in one persistent
  * launch (captured and replayed as a HIP graph when the box set is
- * unchanged).  Asynchronous unless sync != 0. */
+ * unchanged).  NPT boxes that stopped at a volume move in the previous tick
+ * take it first, beside the other boxes, and then move until the same
+ * deadline.  Asynchronous unless sync != 0. */
 int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync);
+/* A box's statistics after the work queued so far (waits for a tick in
+ * flight). */
 int mcgpu_batch_stats(mcgpu_batch* b, int32_t slot, mcgpu_box_stats* out);
+/* The same as of the last tick consumed (by mcgpu_batch_run or any call that
+ * waits), without waiting for a tick in flight: a serving loop launches the
+ * next tick first and then reads, retires and admits while it runs.  No
+ * reference counterpart. */
+int mcgpu_batch_peek(mcgpu_batch* b, int32_t slot, mcgpu_box_stats* out);
 int mcgpu_batch_download(mcgpu_batch* b, int32_t slot, double* x, double* y,
                          double* z);
 /* Copies the recorded trace (if record_trace was set): up to cap moves. */
@@ -332,12 +344,13 @@ int mcgpu_batch_set_autotune(mcgpu_batch* b, int32_t on);
  * 256 / 512 / 1024 lanes (3 per class); n = number of tuned classes. */
 int mcgpu_batch_tuning(mcgpu_batch* b, int64_t* keys, int32_t* threads,
                        double* moves_per_tick, int32_t cap, int32_t* n);
-/* Host seconds spent so far waiting for ticks to finish, in NPT volume
- * moves (their batched passes included), and launching ticks. */
+/* Host seconds spent so far waiting for ticks to finish, queueing NPT
+ * volume moves, and launching ticks. */
 int mcgpu_batch_host_seconds(mcgpu_batch* b, double* wait, double* volume,
                              double* launch);
-/* Average device duration (ms) of the batch kernel over the last
- * mcgpu_batch_run calls since the previous query, and their count. */
+/* Average device duration (ms) of the batch kernel's main launch (the
+ * boxes that do not start with a volume move) over the mcgpu_batch_run
+ * calls since the previous query that had one, and their count. */
 int mcgpu_batch_kernel_ms(mcgpu_batch* b, double* avg_ms, int64_t* launches);
 
 #ifdef __cplusplus

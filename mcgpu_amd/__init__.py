@@ -9,10 +9,21 @@ ctypes; there is no Python or CPU fallback.  Build with
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "libmcgpu.so"
 _lib = None
+
+# Hardware queues for the process's HIP streams.  A batch keeps up to seven
+# streams busy at once (a tick's three work-group-size groups; beside them
+# the volume moves and the groups that move after them; admissions).  HIP
+# maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and
+# work on streams that share a queue runs in order, so a volume pass would
+# wait behind a tick.  Raised to this many unless set higher; HIP reads it
+# when its runtime starts, so it takes effect only before the process's
+# first HIP call.
+MIN_HW_QUEUES = 16
 
 MCGPU_OK = 0
 FP64 = 0
@@ -161,6 +172,12 @@ def lib() -> C.CDLL:
         if not _LIB_PATH.exists():
             raise McgpuError(
                 f"{_LIB_PATH} is missing: build it with `python -m mcgpu_amd.build`")
+        try:
+            queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+        except ValueError:
+            queues = 0
+        if queues < MIN_HW_QUEUES:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
         _lib = C.CDLL(str(_LIB_PATH))
         _lib.mcgpu_last_error.restype = C.c_char_p
     return _lib

@@ -378,6 +378,14 @@ class Batch:
         check(lib().mcgpu_batch_stats(self._b, int(slot), C.byref(s)), "mcgpu_batch_stats")
         return s
 
+    def peek(self, slot: int) -> BoxStats:
+        """Statistics as of the last tick consumed, without waiting for a
+        tick in flight (run(sync=False) then peek overlaps host work with
+        the tick)."""
+        s = BoxStats()
+        check(lib().mcgpu_batch_peek(self._b, int(slot), C.byref(s)), "mcgpu_batch_peek")
+        return s
+
     def download(self, slot: int):
         n = self._keep[slot][1]
         x, y, z = np.zeros(n), np.zeros(n), np.zeros(n)

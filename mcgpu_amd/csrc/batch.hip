@@ -8,9 +8,9 @@
 // independent, so a launch is as long as its slowest box.
 //
 // Box memory is carved from one HBM arena (admission by footprint).  NPT
-// volume moves run between launches, decided on the device by a short
-// sequence of multi-box kernels around the full-energy pass (see
-// enqueue_volume_moves).
+// volume moves run on a stream of their own beside a launch, decided on the
+// device by a short sequence of multi-box kernels around the full-energy
+// pass (see begin_volume_moves).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -367,18 +367,23 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, const BatchBox& 
 // pending NPT volume move, or when `budget` wall-clock ticks have passed
 // since the work-group started (0 = no time limit).  Wave 0 draws, builds,
 // decides and commits each move; all waves scan candidates and sum pairs.
-template <int NT>
+// The LATE instance runs the boxes of a tick whose NPT volume move comes
+// first (the volume pass queued just before it on its stream, beside the
+// rest of the tick): their budget counts from *t0, the wall clock the pass
+// stamped when it began, so they stop with the rest of the tick.
+template <int NT, bool LATE>
 __global__ __launch_bounds__(NT) void k_batch(const BatchBox* __restrict__ table,
                                               BoxCtl* ctl,
                                               const int32_t* active,
-                                              int sweeps, long long budget) {
+                                              int sweeps, long long budget,
+                                              const long long* t0) {
   __shared__ BatchShared<NT> S;
   const int t = threadIdx.x;
   const bool w0 = t < 64;
   // active = [count, slot 0, slot 1, ...]; the grid is rounded up
   if ((int)blockIdx.x >= __builtin_amdgcn_readfirstlane(active[0])) return;
   const int slot = __builtin_amdgcn_readfirstlane(active[1 + blockIdx.x]);
-  const long long t_start = wall_clock64();
+  const long long t_start = LATE ? *t0 : wall_clock64();
   // the descriptor is read in place (scalar loads from a table this launch
   // does not write)
   const BatchBox& B = table[slot];
@@ -494,14 +499,17 @@ __device__ void set_geometry_dev(BatchBox& B) {
 }
 
 // NPT volume move, first half, one thread per listed box (semantics: see
-// enqueue_volume_moves): u1 and u2 drawn from the box's own generator, the
+// begin_volume_moves): u1 and u2 drawn from the box's own generator, the
 // new edges and cell geometry set, the scale factor handed to the rescale.
 // V' is the product of the new edges, as acceptVolumeMove reads it
 // (SimulationStep.cpp:476-481); a proposal <= 0 keeps the box (f = 1) and
-// is rejected by the decision.
-__global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k) {
+// is rejected by the decision.  Thread 0 stamps the wall clock into *t0: the
+// start of the tick for the boxes that move after their volume move.
+__global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
+                            long long* t0) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *t0 = wall_clock64();
   if (i >= k) return;
   const int s = items[i].slot;
   BatchBox& B = table[s];
@@ -870,14 +878,28 @@ struct BatchSlot {
   // counters before this run (a box added from a resume record)
   int64_t base_steps = 0, base_acc = 0, base_vol = 0, base_vacc = 0;
   bool dirty = false;      // the device's control block is newer than h_ctl
-  bool vol_queued = false; // in the volume pass queued at the last boundary
+  bool late = false;       // runs after its volume move in the tick in flight
 };
 
-// Host side of a batch.  Streams: `stream` carries the ticks (group 0 of the
-// graph), the read-back after them and, between ticks, the NPT volume moves
-// (decided on the device, so the host never waits for them); `side[]` the
-// other work-group-size groups of a tick; `sstream` admissions (their
-// uploads, cells and start energy), which every tick waits for (ev_ready).
+// One captured tick launch: the persistent kernel of each work-group-size
+// group over its active list, side by side; replayed while the grid sizes,
+// sweep count and time budget are unchanged.
+struct TickGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int sweeps = -1;
+  uint64_t version = 0;
+  int grid[kGroups] = {};
+};
+
+// Host side of a batch.  Streams: `stream` carries the ticks and the
+// read-back after them; `vstream`, beside each tick, the NPT volume moves of
+// the boxes that reached a volume stop in the previous one (decided on the
+// device) and then those boxes' own share of the tick (the late graph), which
+// the read-back waits for (ev_vdone); `side[]` serve graph capture (the
+// work-group-size groups of a tick run side by side); `sstream` admissions
+// (their uploads, cells and start energy), which every tick waits for
+// (ev_ready).
 // Ownership: control blocks and generator states are the device's; the host
 // mirrors them from each tick's read-back (merged for the boxes the tick
 // ran) and writes a box's only when it sets the box up.
@@ -885,10 +907,11 @@ struct mcgpu_batch {
   hipStream_t stream = nullptr;
   hipStream_t side[kGroups - 1] = {};
   hipStream_t sstream = nullptr;
+  hipStream_t vstream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipEvent_t join[kGroups - 1] = {};
   hipEvent_t ev_ready = nullptr;  // set-up work queued so far is done
-  hipEvent_t ev_vol = nullptr;    // end of the last queued volume pass
+  hipEvent_t ev_vdone = nullptr;  // the volume branch of the tick is done
   Arena arena;
   int32_t max_boxes = 0;
   int device = 0;  // the device current when the batch was created
@@ -907,24 +930,24 @@ struct mcgpu_batch {
   Rng31* d_rng = nullptr;
   Rng31* h_rng = nullptr;       // pinned mirror
   Rng31* h_rng_rb = nullptr;    // pinned read-back of a tick
-  int32_t* d_active = nullptr;  // [kGroups * (max_boxes + 1)]: per group
-  int32_t* h_active = nullptr;  // pinned     [count, slots...]
+  // [2 * kGroups * (max_boxes + 1)]: per group [count, slots...], the
+  // main lists then the late ones
+  int32_t* d_active = nullptr;
+  int32_t* h_active = nullptr;  // pinned
   std::vector<int32_t> next_active;  // the lists of the tick being set up
   std::vector<int> eff_group;        // work-group size of each box this tick
   VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
   VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
   double* d_volout = nullptr;   // [2 * max_boxes] energies of a volume pass
+  long long* d_t0 = nullptr;    // wall clock at the start of the volume pass
   long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
   double clock_khz = 100000.0;
   bool pending = false;         // a tick in flight
   std::vector<int> launched;    // slots of the tick in flight
-  int n_active[kGroups] = {};
-  // graph cache
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int graph_sweeps = -1;
-  uint64_t graph_version = 0, version = 1;
+  std::vector<int> late;        // of which after their volume move
+  TickGraph main_graph, late_graph;
+  uint64_t version = 1;  // bumped when the time budget changes
   // work-group-size tuner: per box class, moves per full tick at each size
   struct Tune {
     int best = -1;
@@ -1023,7 +1046,8 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 }
 
 // NPT volume moves of the boxes in `pend`, batched and decided on the
-// device, queued between two ticks: u1, u2 from each box's generator, the
+// device, queued on `st` at the start of the tick in which those boxes move
+// on (beside the tick's other boxes): u1, u2 from each box's generator, the
 // rescale (with backup), cells for the new geometry, the full energy, the
 // decision, and the undo of rejected moves.  Semantics (the build's own;
 // the reference's NPT path, Ensemble.cpp:9-28 / SimulationStep.cpp:476-518,
@@ -1037,44 +1061,44 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 //   exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2 (SimulationStep.cpp:
 //   476-488), and then the running energy becomes E' exactly.  A proposal
 //   V' <= 0 is rejected.
-int enqueue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend) {
+// Queued in two parts, so that the rest of the tick can be launched between
+// them: the first draws u1, u2 and stamps the clock (the late boxes' tick
+// start), the second rescales, evaluates, decides and undoes.
+int begin_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
+  const int K = (int)pend.size();
+  for (int k = 0; k < K; k++) b->h_items[k] = VolItem{pend[k], 0, 1.0};
+  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * K,
+                               hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_vol_begin, dim3((K + 63) / 64), dim3(64), 0, st, b->d_table, b->d_ctl,
+                     b->d_items, K, b->d_t0);
+  MCGPU_HIP_TRY(hipGetLastError());
+  return MCGPU_OK;
+}
+
+int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
   int rc;
   const int K = (int)pend.size();
   int max_mol = 0, max_cells = 0;
   bool any_fp64 = false, any_mixed = false;
-  for (int k = 0; k < K; k++) {
-    const int s = pend[k];
-    b->h_items[k] = VolItem{s, 0, 1.0};
+  for (int s : pend) {
     (b->slots[s].params.precision == MCGPU_MIXED ? any_mixed : any_fp64) = true;
     max_mol = std::max(max_mol, b->h_table[s].bv.n_mol);
     max_cells = std::max(max_cells, b->slots[s].lay.ncell);
-    // the decision puts the box back to running (it is below its target)
-    b->h_ctl[s].status = kRunning;
   }
-  const hipStream_t st = b->stream;
-  MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * K,
-                               hipMemcpyHostToDevice, st));
-  const int tb = (K + 63) / 64;
-  hipLaunchKernelGGL(k_vol_begin, dim3(tb), dim3(64), 0, st, b->d_table, b->d_ctl, b->d_items,
-                     K);
   hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
   if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st)) ||
       (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st, any_fp64, any_mixed)))
     return rc;
-  hipLaunchKernelGGL(k_vol_decide, dim3(tb), dim3(64), 0, st, b->d_table, b->d_ctl,
+  hipLaunchKernelGGL(k_vol_decide, dim3((K + 63) / 64), dim3(64), 0, st, b->d_table, b->d_ctl,
                      b->d_items, K, (const double*)b->d_volout);
   hipLaunchKernelGGL(k_restore_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
   // cells again for the restored boxes (a rebuild of an accepted box's
   // lists reproduces them: same positions and geometry, ordered slots)
-  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st))) return rc;
-  // a removal of one of these boxes before the pass ends waits for it
-  MCGPU_HIP_TRY(hipEventRecord(b->ev_vol, st));
-  for (int k = 0; k < K; k++) b->slots[pend[k]].vol_queued = true;
-  return MCGPU_OK;
+  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st);
 }
 
 // Work-group-size tuner.  Per box class (neighbour source, pair precision,
@@ -1090,7 +1114,8 @@ int64_t tune_key_of(const mcgpu_box_desc& d, const mcgpu_run_params& p, int max_
 
 void tune_sample(mcgpu_batch* b, int s) {
   BatchSlot& S = b->slots[s];
-  if (S.trial < 0) return;
+  // a box that took its volume move first ran a shortened tick
+  if (S.trial < 0 || S.late) return;
   const BoxCtl& c = b->h_ctl[s];
   const int64_t moved = c.steps_done - S.steps_at_launch;
   // only full ticks measure speed: the box ran until the time budget
@@ -1120,22 +1145,21 @@ void tune_sample(mcgpu_batch* b, int s) {
 }
 
 // Tick boundary: consumes a finished tick (its read-back merged for the
-// boxes it ran) and queues the volume moves of the boxes that reached a
-// volume stop in it, ahead of the next tick, in which those boxes run
-// again.  Without a tick in flight it does nothing.
+// boxes it ran, its volume branch included).  Boxes that reached a volume
+// stop in it stay at kVolumePending until the next tick takes their volume
+// move.  Without a tick in flight it does nothing.
 int batch_boundary(mcgpu_batch* b) {
   if (!b->pending) return MCGPU_OK;
   const double t0 = host_now();
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
   b->t_wait += host_now() - t0;
   b->pending = false;
-  // every volume pass queued before this tick has ended with it
-  for (BatchSlot& S : b->slots) S.vol_queued = false;
   for (int s : b->launched) {
     b->h_ctl[s] = b->h_ctl_rb[s];
     b->h_rng[s] = b->h_rng_rb[s];
     b->slots[s].dirty = false;
     if (b->budget_ticks > 0) tune_sample(b, s);
+    b->slots[s].late = false;
   }
   if (b->timing_pending) {
     float ms = 0.f;
@@ -1144,15 +1168,14 @@ int batch_boundary(mcgpu_batch* b) {
     b->kernel_launches++;
     b->timing_pending = false;
   }
-  std::vector<int> pend;
-  for (int s : b->launched)
-    if (b->slots[s].used && b->h_ctl[s].status == kVolumePending) pend.push_back(s);
   b->launched.clear();
-  if (pend.empty()) return MCGPU_OK;
-  const double t1 = host_now();
-  const int rc = enqueue_volume_moves(b, pend);
-  b->t_volume += host_now() - t1;
-  return rc;
+  b->late.clear();
+  return MCGPU_OK;
+}
+
+// Whether the tick in flight runs slot s.
+bool in_flight(const mcgpu_batch* b, int s) {
+  return b->pending && std::find(b->launched.begin(), b->launched.end(), s) != b->launched.end();
 }
 
 // Control blocks of slots set up since their last tick, from the device.
@@ -1174,6 +1197,7 @@ int batch_quiesce(mcgpu_batch* b) {
   int rc;
   if ((rc = batch_boundary(b))) return rc;
   MCGPU_HIP_TRY(hipStreamSynchronize(b->stream));
+  MCGPU_HIP_TRY(hipStreamSynchronize(b->vstream));
   MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
   // nothing in flight: the mirrors take the device's state for every box
   MCGPU_HIP_TRY(hipMemcpy(b->h_ctl, b->d_ctl, sizeof(BoxCtl) * b->max_boxes,
@@ -1226,8 +1250,9 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
             hipSuccess &&
         cus > 0)
       b->n_cu = cus;
-    const void* kern[kGroups] = {(const void*)k_batch<256>, (const void*)k_batch<512>,
-                                 (const void*)k_batch<1024>};
+    const void* kern[kGroups] = {(const void*)k_batch<256, false>,
+                                 (const void*)k_batch<512, false>,
+                                 (const void*)k_batch<1024, false>};
     for (int g = 0; g < kGroups; g++) {
       int per_cu = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern[g], kGroupNT[g], 0) ==
@@ -1244,26 +1269,28 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
   const size_t M = max_boxes;
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&b->sstream, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->vstream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
             hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_ready, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->ev_vol, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&b->ev_vdone, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; ok && g < kGroups - 1; g++)
     ok = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&b->join[g], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipMalloc(&b->d_table, sizeof(BatchBox) * M) == hipSuccess &&
        hipMalloc(&b->d_ctl, sizeof(BoxCtl) * M) == hipSuccess &&
        hipMalloc(&b->d_rng, sizeof(Rng31) * M) == hipSuccess &&
-       hipMalloc(&b->d_active, sizeof(int32_t) * kGroups * (M + 1)) == hipSuccess &&
+       hipMalloc(&b->d_active, sizeof(int32_t) * 2 * kGroups * (M + 1)) == hipSuccess &&
        hipMalloc(&b->d_items, sizeof(VolItem) * M) == hipSuccess &&
        hipMalloc(&b->d_one, sizeof(VolItem) * M) == hipSuccess &&
        hipMalloc(&b->d_volout, sizeof(double) * 2 * M) == hipSuccess &&
+       hipMalloc(&b->d_t0, sizeof(long long)) == hipSuccess &&
        hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_ctl_rb, sizeof(BoxCtl) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_rng, sizeof(Rng31) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_rng_rb, sizeof(Rng31) * M, 0) == hipSuccess &&
-       hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * kGroups * (M + 1), 0) ==
+       hipHostMalloc((void**)&b->h_active, sizeof(int32_t) * 2 * kGroups * (M + 1), 0) ==
            hipSuccess &&
        hipHostMalloc((void**)&b->h_items, sizeof(VolItem) * M, 0) == hipSuccess;
   if (!ok) {
@@ -1274,8 +1301,8 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
   std::memset(b->h_ctl, 0, sizeof(BoxCtl) * M);
   // empty active lists on both sides: a tick uploads a group's list only
   // when it differs from what the device holds (h_active mirrors d_active)
-  std::memset(b->h_active, 0, sizeof(int32_t) * kGroups * (M + 1));
-  if (hipMemset(b->d_active, 0, sizeof(int32_t) * kGroups * (M + 1)) != hipSuccess) {
+  std::memset(b->h_active, 0, sizeof(int32_t) * 2 * kGroups * (M + 1));
+  if (hipMemset(b->d_active, 0, sizeof(int32_t) * 2 * kGroups * (M + 1)) != hipSuccess) {
     set_error("mcgpu_batch_create: allocation failed");
     return fail(MCGPU_ERR_HIP);
   }
@@ -1298,26 +1325,23 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
 
 extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
   if (!b) return MCGPU_OK;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (hipStream_t s : b->side)
+  for (hipStream_t s : {b->stream, b->vstream, b->side[0], b->side[1], b->sstream})
     if (s) (void)hipStreamSynchronize(s);
-
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->sstream) (void)hipStreamSynchronize(b->sstream);
+  for (TickGraph* G : {&b->main_graph, &b->late_graph}) {
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+  }
   for (void* p : {(void*)b->d_table, (void*)b->d_ctl, (void*)b->d_rng, (void*)b->d_active,
-                  (void*)b->d_items, (void*)b->d_one, (void*)b->d_volout})
+                  (void*)b->d_items, (void*)b->d_one, (void*)b->d_volout, (void*)b->d_t0})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->h_table, (void*)b->h_ctl, (void*)b->h_ctl_rb, (void*)b->h_rng,
                   (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_items})
     if (p) (void)hipHostFree(p);
-  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_ready, b->ev_vol})
+  for (hipEvent_t e : {b->ev0, b->ev1, b->fork, b->ev_ready, b->ev_vdone})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->join)
     if (e) (void)hipEventDestroy(e);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
-  if (b->sstream) (void)hipStreamDestroy(b->sstream);
-  for (hipStream_t s : b->side)
+  for (hipStream_t s : {b->stream, b->vstream, b->side[0], b->side[1], b->sstream})
     if (s) (void)hipStreamDestroy(s);
   delete b;
   return MCGPU_OK;
@@ -1625,14 +1649,12 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
     set_error("mcgpu_batch_remove: bad slot");
     return MCGPU_ERR_RANGE;
   }
-  int rc = batch_boundary(b);
-  if (rc) return rc;
+  // a tick in flight without this box touches neither its control block nor
+  // its arena block (its read-back copies the block but merges launched
+  // slots only), so only a tick that runs it is waited for
+  int rc;
+  if (in_flight(b, s) && (rc = batch_boundary(b))) return rc;
   if (b->slots[s].dirty) MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
-  // a box at its volume stop has a volume pass queued on the tick stream
-  // that still writes its control block and arena block: the set-up stream,
-  // which carries this slot's reset and every later upload into the freed
-  // block, waits for that pass
-  if (b->slots[s].vol_queued) MCGPU_HIP_TRY(hipStreamWaitEvent(b->sstream, b->ev_vol, 0));
   b->h_ctl[s] = BoxCtl{};
   b->h_ctl[s].status = kDone;
   if ((rc = upload_ctl(b, s, b->sstream))) return rc;
@@ -1640,29 +1662,107 @@ extern "C" int mcgpu_batch_remove(mcgpu_batch* b, int32_t s) {
   return MCGPU_OK;
 }
 
-// One tick: every running box (not in a volume move) advances up to
-// `sweeps` moves in one graph launch.  The previous tick is consumed first;
-// with sync the call returns after this tick is consumed too (the volume
-// pass it started then runs beside the next tick).
+namespace {
+
+// Replays G on `st` (the late graph on the volume stream, after the volume
+// pass), capturing it again when the grids, sweep count or time budget
+// changed; e0 and e1 (optional) are recorded around the replay.  Grids are
+// rounded up (surplus work-groups exit at once), so a graph survives most
+// admissions and retirements.  The late graph keeps every group at 32
+// work-groups or more (empty ones exit at once): its box set changes every
+// tick.
+int launch_tick_graph(mcgpu_batch* b, TickGraph& G, bool late, hipStream_t st,
+                      const int n[kGroups], int sweeps, hipEvent_t e0, hipEvent_t e1) {
+  const int stride = b->max_boxes + 1;
+  const int32_t* act = b->d_active + (late ? kGroups * stride : 0);
+  int grid[kGroups];
+  bool same = G.exec && G.version == b->version && G.sweeps == sweeps;
+  for (int g = 0; g < kGroups; g++) {
+    const int want = late ? std::max(n[g], 1) : n[g];
+    grid[g] = want ? std::min(b->max_boxes, (want + 31) & ~31) : 0;
+    if (late && G.exec && G.grid[g] >= grid[g]) grid[g] = G.grid[g];
+    same = same && G.grid[g] == grid[g];
+  }
+  if (!same) {
+    if (G.exec) MCGPU_HIP_TRY(hipGraphExecDestroy(G.exec));
+    if (G.graph) MCGPU_HIP_TRY(hipGraphDestroy(G.graph));
+    G.exec = nullptr;
+    G.graph = nullptr;
+    hipStream_t on[kGroups] = {st, b->side[0], b->side[1]};
+    MCGPU_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    MCGPU_HIP_TRY(hipEventRecord(b->fork, st));
+    for (int g = 1; g < kGroups; g++)
+      MCGPU_HIP_TRY(hipStreamWaitEvent(b->side[g - 1], b->fork, 0));
+    for (int g = 0; g < kGroups; g++) {
+      if (!grid[g]) continue;
+      using K = void (*)(const BatchBox*, BoxCtl*, const int32_t*, int, long long,
+                         const long long*);
+      const K k[2][kGroups] = {{k_batch<256, false>, k_batch<512, false>, k_batch<1024, false>},
+                               {k_batch<256, true>, k_batch<512, true>, k_batch<1024, true>}};
+      hipLaunchKernelGGL(k[late][g], dim3(grid[g]), dim3(kGroupNT[g]), 0, on[g], b->d_table,
+                         b->d_ctl, act + g * stride, sweeps, b->budget_ticks,
+                         (const long long*)b->d_t0);
+    }
+    for (int g = 1; g < kGroups; g++) {
+      MCGPU_HIP_TRY(hipEventRecord(b->join[g - 1], b->side[g - 1]));
+      MCGPU_HIP_TRY(hipStreamWaitEvent(st, b->join[g - 1], 0));
+    }
+    MCGPU_HIP_TRY(hipStreamEndCapture(st, &G.graph));
+    MCGPU_HIP_TRY(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.version = b->version;
+    G.sweeps = sweeps;
+    for (int g = 0; g < kGroups; g++) G.grid[g] = grid[g];
+  }
+  if (e0) MCGPU_HIP_TRY(hipEventRecord(e0, st));
+  MCGPU_HIP_TRY(hipGraphLaunch(G.exec, st));
+  if (e1) MCGPU_HIP_TRY(hipEventRecord(e1, st));
+  return MCGPU_OK;
+}
+
+// Uploads the active lists of one side (main or late) that changed.  The
+// previous tick, and so its uploads from h_active, has completed (the
+// boundary waited for it).
+int upload_lists(mcgpu_batch* b, bool late, hipStream_t st) {
+  const int stride = b->max_boxes + 1;
+  const size_t base = late ? (size_t)kGroups * stride : 0;
+  for (int g = 0; g < kGroups; g++) {
+    const int32_t* want = &b->next_active[base + g * stride];
+    const size_t words = 1 + (size_t)want[0];
+    int32_t* h = b->h_active + base + g * stride;
+    if (std::memcmp(h, want, sizeof(int32_t) * words) == 0) continue;
+    std::memcpy(h, want, sizeof(int32_t) * words);
+    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + base + g * stride, h, sizeof(int32_t) * words,
+                                 hipMemcpyHostToDevice, st));
+  }
+  return MCGPU_OK;
+}
+
+}  // namespace
+
+// One tick: every running box advances up to `sweeps` moves in one graph
+// launch; beside it, on the volume stream, the boxes that reached a volume
+// stop in the previous tick take their volume move and then advance in the
+// late graph until the same deadline.  The previous tick is consumed first;
+// with sync the call returns after this tick is consumed too.
 extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   if (!b || sweeps < 1) return MCGPU_ERR_INVALID;
   int rc = batch_boundary(b);
   if (rc) return rc;
   const int stride = b->max_boxes + 1;
-  int n[kGroups] = {};
-  int total = 0;
   b->launched.clear();
-  std::vector<int32_t>& act = b->next_active;
-  act.assign((size_t)kGroups * stride, 0);
+  b->late.clear();
   for (int s = 0; s < b->max_boxes; s++) {
     const BatchSlot& S = b->slots[s];
-    if (S.used && b->h_ctl[s].status == kRunning) {
-      b->launched.push_back(s);
-      b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
+    const int32_t st = b->h_ctl[s].status;
+    if (!S.used || (st != kRunning && st != kVolumePending)) continue;
+    b->launched.push_back(s);
+    b->slots[s].steps_at_launch = b->h_ctl[s].steps_done;
+    if (st == kVolumePending) {
+      b->late.push_back(s);
+      b->slots[s].late = true;
     }
   }
-  total = (int)b->launched.size();
-  if (total == 0) return MCGPU_OK;
+  if (b->launched.empty()) return MCGPU_OK;
   // Work-group sizes of this tick: each box's preferred one, unless the
   // boxes need more compute units than the device has.  Then every box
   // takes 256 lanes, whose work-groups share CUs, so the whole tick is
@@ -1682,66 +1782,49 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   if (demand > b->n_cu)
     for (int s : b->launched)
       if (b->slots[s].trial < 0) eff[s] = 0;
-  for (int s : b->launched) act[eff[s] * stride + 1 + n[eff[s]]++] = s;
+  std::vector<int32_t>& act = b->next_active;
+  act.assign((size_t)2 * kGroups * stride, 0);
+  int n[2][kGroups] = {};
+  for (int s : b->launched) {
+    const int side = b->slots[s].late ? 1 : 0;
+    const size_t at = (size_t)(side * kGroups + eff[s]) * stride;
+    act[at + 1 + n[side][eff[s]]++] = s;
+  }
+  for (int side = 0; side < 2; side++)
+    for (int g = 0; g < kGroups; g++) act[(size_t)(side * kGroups + g) * stride] = n[side][g];
   const double tl = host_now();
-  // set-up work queued so far (admissions, decided volume moves) first
+  // set-up work queued so far (admissions) first
   MCGPU_HIP_TRY(hipEventRecord(b->ev_ready, b->sstream));
+  const bool any_main = b->late.size() < b->launched.size();
+  const bool any_late = !b->late.empty();
+  double tv = 0.0;
+  // Order: the volume branch's first kernel (its clock stamp), the main
+  // graph, then the rest of the branch, so that both sides start together
+  // and neither waits for the host's queueing of the other.
+  if (any_late) {
+    MCGPU_HIP_TRY(hipStreamWaitEvent(b->vstream, b->ev_ready, 0));
+    if ((rc = begin_volume_moves(b, b->late, b->vstream))) return rc;
+  }
   MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_ready, 0));
-  // a group's list goes up only when it changed (the previous tick, and so
-  // its upload from h_active, has completed: batch_boundary waited for it)
-  for (int g = 0; g < kGroups; g++) {
-    act[g * stride] = n[g];
-    const size_t words = 1 + (size_t)n[g];
-    int32_t* h = b->h_active + g * stride;
-    if (std::memcmp(h, &act[g * stride], sizeof(int32_t) * words) == 0) continue;
-    std::memcpy(h, &act[g * stride], sizeof(int32_t) * words);
-    MCGPU_HIP_TRY(hipMemcpyAsync(b->d_active + g * stride, h, sizeof(int32_t) * words,
-                                 hipMemcpyHostToDevice, b->stream));
+  if (any_main &&
+      ((rc = upload_lists(b, false, b->stream)) ||
+       (rc = launch_tick_graph(b, b->main_graph, false, b->stream, n[0], sweeps, b->ev0,
+                               b->ev1))))
+    return rc;
+  if (any_late) {
+    const double t1 = host_now();
+    if ((rc = finish_volume_moves(b, b->late, b->vstream))) return rc;
+    tv = host_now() - t1;
+    b->t_volume += tv;
+    if ((rc = upload_lists(b, true, b->vstream)) ||
+        (rc = launch_tick_graph(b, b->late_graph, true, b->vstream, n[1], sweeps, nullptr,
+                                nullptr)))
+      return rc;
+    MCGPU_HIP_TRY(hipEventRecord(b->ev_vdone, b->vstream));
+    MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_vdone, 0));
   }
-  // grids rounded up (surplus work-groups exit at once), so the captured
-  // graph survives most admissions and retirements
-  int grid[kGroups];
-  bool same = b->gexec && b->graph_version == b->version && b->graph_sweeps == sweeps;
-  for (int g = 0; g < kGroups; g++) {
-    grid[g] = n[g] ? std::min(b->max_boxes, (n[g] + 31) & ~31) : 0;
-    same = same && b->n_active[g] == grid[g];
-  }
-  if (!same) {
-    if (b->gexec) MCGPU_HIP_TRY(hipGraphExecDestroy(b->gexec));
-    if (b->graph) MCGPU_HIP_TRY(hipGraphDestroy(b->graph));
-    b->gexec = nullptr;
-    b->graph = nullptr;
-    // One tick = the persistent launch of each work-group-size group, side
-    // by side on their own streams; captured once and replayed while the
-    // grid sizes, sweep count and time budget are unchanged.
-    MCGPU_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    MCGPU_HIP_TRY(hipEventRecord(b->fork, b->stream));
-    for (int g = 1; g < kGroups; g++)
-      MCGPU_HIP_TRY(hipStreamWaitEvent(b->side[g - 1], b->fork, 0));
-    if (grid[0])
-      hipLaunchKernelGGL(k_batch<256>, dim3(grid[0]), dim3(256), 0, b->stream, b->d_table,
-                         b->d_ctl, b->d_active, sweeps, b->budget_ticks);
-    if (grid[1])
-      hipLaunchKernelGGL(k_batch<512>, dim3(grid[1]), dim3(512), 0, b->side[0], b->d_table,
-                         b->d_ctl, b->d_active + stride, sweeps, b->budget_ticks);
-    if (grid[2])
-      hipLaunchKernelGGL(k_batch<1024>, dim3(grid[2]), dim3(1024), 0, b->side[1], b->d_table,
-                         b->d_ctl, b->d_active + 2 * stride, sweeps, b->budget_ticks);
-    for (int g = 1; g < kGroups; g++) {
-      MCGPU_HIP_TRY(hipEventRecord(b->join[g - 1], b->side[g - 1]));
-      MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->join[g - 1], 0));
-    }
-    MCGPU_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
-    MCGPU_HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-    b->graph_version = b->version;
-    b->graph_sweeps = sweeps;
-    for (int g = 0; g < kGroups; g++) b->n_active[g] = grid[g];
-  }
-  MCGPU_HIP_TRY(hipEventRecord(b->ev0, b->stream));
-  MCGPU_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
-  MCGPU_HIP_TRY(hipEventRecord(b->ev1, b->stream));
   // read-back of the launched boxes' control blocks and generators into
-  // staging (slots below the highest launched one; the boundary merges the
+  // staging (slots up to the highest launched one; the boundary merges the
   // launched slots only)
   const size_t nrb = (size_t)b->launched.back() + 1;
   MCGPU_HIP_TRY(hipMemcpyAsync(b->h_ctl_rb, b->d_ctl, sizeof(BoxCtl) * nrb,
@@ -1749,8 +1832,8 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   MCGPU_HIP_TRY(hipMemcpyAsync(b->h_rng_rb, b->d_rng, sizeof(Rng31) * nrb,
                                hipMemcpyDeviceToHost, b->stream));
   b->pending = true;
-  b->timing_pending = true;
-  b->t_launch += host_now() - tl;
+  b->timing_pending = any_main;
+  b->t_launch += host_now() - tl - tv;
   if (sync) return batch_boundary(b);
   return MCGPU_OK;
 }
@@ -1764,13 +1847,10 @@ extern "C" int mcgpu_batch_host_seconds(mcgpu_batch* b, double* wait, double* vo
   return MCGPU_OK;
 }
 
-extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out) {
-  if (!b || !out || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
-    set_error("mcgpu_batch_stats: bad slot");
-    return MCGPU_ERR_RANGE;
-  }
-  int rc = batch_boundary(b);
-  if (rc || (rc = refresh_dirty(b, s))) return rc;
+namespace {
+
+// A slot's statistics from its host mirror.
+void stats_from_mirror(const mcgpu_batch* b, int s, mcgpu_box_stats* out) {
   const BoxCtl& c = b->h_ctl[s];
   out->steps_done = c.steps_done;
   out->accepted = c.accepted;
@@ -1784,6 +1864,42 @@ extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out
   for (int k = 0; k < 3; k++) out->box[k] = c.L[k];
   out->done = c.status == kDone;
   out->fault = c.fault ? c.fault : b->slots[s].fault;
+}
+
+}  // namespace
+
+extern "C" int mcgpu_batch_stats(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out) {
+  if (!b || !out || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
+    set_error("mcgpu_batch_stats: bad slot");
+    return MCGPU_ERR_RANGE;
+  }
+  int rc = batch_boundary(b);
+  if (rc || (rc = refresh_dirty(b, s))) return rc;
+  stats_from_mirror(b, s, out);
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_batch_peek(mcgpu_batch* b, int32_t s, mcgpu_box_stats* out) {
+  if (!b || !out || s < 0 || s >= b->max_boxes || !b->slots[s].used) {
+    set_error("mcgpu_batch_peek: bad slot");
+    return MCGPU_ERR_RANGE;
+  }
+  if (b->slots[s].dirty) {
+    if (!in_flight(b, s)) {
+      int rc = refresh_dirty(b, s);
+      if (rc) return rc;
+    } else {
+      // admitted just before the tick in flight, which writes its control
+      // block: only the start energy is read (after the set-up stream, which
+      // computes it; the tick stores it back unchanged), and the box is
+      // reported as before its first move
+      MCGPU_HIP_TRY(hipStreamSynchronize(b->sstream));
+      MCGPU_HIP_TRY(hipMemcpy(&b->h_ctl[s].start_energy, &b->d_ctl[s].start_energy,
+                              sizeof(double), hipMemcpyDeviceToHost));
+      b->h_ctl[s].energy = b->h_ctl[s].start_energy;
+    }
+  }
+  stats_from_mirror(b, s, out);
   return MCGPU_OK;
 }
 

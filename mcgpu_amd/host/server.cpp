@@ -1,11 +1,12 @@
 // Serving loop: continuous batching of simulation jobs over one device batch.
 //
-// Jobs arrive over time (arrival_s on the server clock).  Between persistent
-// launches ("ticks") the loop admits every arrived job whose device
-// footprint fits the HBM budget and a free slot (FIFO by arrival), then
-// launches one tick that advances every resident box by up to
-// sweeps_per_tick moves, then retires finished boxes, freeing their slot and
-// memory for the next admissions.  Box state and RNG stay on the device from
+// Jobs arrive over time (arrival_s on the server clock).  Each pass of the
+// loop waits for the persistent launch in flight ("tick") and launches the
+// next, which advances every resident box by up to sweeps_per_tick moves;
+// while it runs, the loop retires the boxes the finished tick completed
+// (freeing their slot and memory) and admits every arrived job whose device
+// footprint fits the HBM budget and a free slot (FIFO by arrival); they run
+// from the following tick on.  Box state and RNG stay on the device from
 // admission to retirement.  Latency of a job = finish time - arrival time.
 //
 // The loop runs on a host thread of the server (mcgpu_server_start), which
@@ -167,18 +168,17 @@ int serve_loop(mcgpu_server* s) {
         s->cv_loop.wait_for(lock, std::chrono::duration<double>(wait));
       continue;
     }
-    // The tick runs asynchronously: jobs that arrive meanwhile are admitted
-    // (host set-up and queued uploads) while it runs; retirement below
-    // waits for it.
+    // Pipelined ticks: the run call consumes the previous tick (waits for
+    // it) and launches the next one.  While that one runs, the jobs the
+    // consumed tick finished are retired (peek reads the statistics as of
+    // the consumed tick, and removing a box that the running tick does not
+    // run waits for nothing) and arrived jobs are admitted (host set-up and
+    // queued uploads), so the device idles only for the consume-and-launch
+    // step between ticks.
     const double tt = s->now();
     int rc = mcgpu_batch_run(s->batch, s->cfg.sweeps_per_tick, 0);
     if (rc != MCGPU_OK) return rc;
     t_run += s->now() - tt;
-    {
-      const double t2 = s->now();
-      admit(false);
-      t_admit += s->now() - t2;
-    }
     last_progress = s->now();
     if (log_every > 0 && s->now() - tt > log_every) {
       std::fprintf(stderr, "[server] slow tick: %.2fs with %zu resident\n", s->now() - tt,
@@ -211,7 +211,7 @@ int serve_loop(mcgpu_server* s) {
         std::lock_guard<std::mutex> lock(s->mu);
         slot = s->jobs[id].slot;
       }
-      rc = mcgpu_batch_stats(s->batch, slot, &st);
+      rc = mcgpu_batch_peek(s->batch, slot, &st);
       if (rc != MCGPU_OK) return rc;
       std::unique_lock<std::mutex> lock(s->mu);
       Job& j = s->jobs[id];
@@ -248,6 +248,11 @@ int serve_loop(mcgpu_server* s) {
       s->cv_done.notify_all();
     }
     t_retire += s->now() - tr;
+    {
+      const double t2 = s->now();
+      admit(false);
+      t_admit += s->now() - t2;
+    }
     std::lock_guard<std::mutex> lock(s->mu);
     s->stats.ticks++;
     s->stats.total_moves += moved;

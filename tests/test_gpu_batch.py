@@ -133,6 +133,56 @@ def test_batch_admission_by_footprint():
     assert b.stats(s3).steps_done == 100 and b.stats(s2).steps_done == 100
 
 
+def test_batch_pipelined_ticks_peek_remove_and_admit():
+    """The serving loop's order: launch a tick without waiting, read the
+    consumed tick's statistics (peek), remove a box that the running tick
+    does not run and admit another into the freed memory while the tick
+    runs.  Every box must still equal its own oracle run."""
+    import ctypes as C
+    hb = load("water256")
+    d = hb.desc()
+
+    def need(steps):
+        p = m.RunParams(1, 0, 0, 0, steps, 0.0, 0.0, 0.0, 0, 1)
+        n = C.c_size_t()
+        m.check(m.lib().mcgpu_box_footprint(C.byref(d), C.byref(p), C.byref(n)))
+        return n.value
+
+    def oracle(seed, steps):
+        return O.OracleBox(hb.arrays()).run_nvt(seed, steps, use_libc=False)
+
+    b = m.Batch(4, hbm_budget=need(900) + int(1.5 * need(300)))
+    s1 = b.add(hb, steps=300, seed=1, record_trace=True)
+    s2 = b.add(hb, steps=900, seed=7, record_trace=True)
+    with pytest.raises(m.McgpuError, match="budget"):
+        b.add(hb, steps=300, seed=3, record_trace=True)
+    b.run(300, sync=False)  # tick 1: box 1 finishes
+    fin1, e01, nacc1, _ = oracle(1, 300)
+    p = b.peek(s1)  # admitted just before the tick in flight
+    assert p.steps_done == 0 and not p.done
+    assert p.start_energy == p.energy == pytest.approx(e01, rel=1e-11)
+    b.run(300, sync=False)  # consumes tick 1, launches tick 2 (box 2 only)
+    p1, p2 = b.peek(s1), b.peek(s2)
+    assert p1.done and p1.steps_done == 300 and p2.steps_done == 300 and not p2.done
+    assert p1.accepted == nacc1
+    assert p1.energy == pytest.approx(fin1, rel=1e-11, abs=1e-7)
+    b.remove(s1)
+    s3 = b.add(hb, steps=300, seed=3, record_trace=True)
+    p3 = b.peek(s3)
+    assert p3.steps_done == 0 and p3.start_energy == pytest.approx(e01, rel=1e-11)
+    st2, st3 = run_all(b, [s2, s3], 900, 300)
+    assert st2.steps_done == 900 and st3.steps_done == 300
+    for s, seed, steps in ((s2, 7, 900), (s3, 3, 300)):
+        fin, e0, nacc, t = oracle(seed, steps)
+        tr = b.trace(s)
+        assert np.array_equal(tr.mol, t["mol"]) and np.array_equal(tr.accepted, t["accepted"])
+        assert de_ok(tr.de, t)
+        st = b.stats(s)
+        assert st.accepted == nacc
+        assert st.energy == pytest.approx(fin, rel=1e-11, abs=1e-7)
+    b.close()
+
+
 def test_batch_cells_equal_brute_over_boundary_crossings():
     """A box with 8 cells per dimension run long enough that many moves
     carry a molecule's primary across the periodic boundary (the cell range
