@@ -668,29 +668,38 @@ __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
   }
 }
 
-// Cell lists of every listed box in cell mode, rebuilt for its current
-// geometry: clear, assign, order (see cells.h).
+// Cell lists of every listed box in cell mode (with undone_only, of the
+// boxes whose volume move was undone), rebuilt for its current geometry:
+// clear, assign, order (see cells.h).
+__device__ __forceinline__ bool cells_skip(const BoxView& bv, const VolItem& it,
+                                           int undone_only) {
+  return bv.mode != MCGPU_NB_CELL || (undone_only && !it.undo);
+}
+
 __global__ __launch_bounds__(256) void k_cells_clear_multi(const BatchBox* table,
-                                                           const VolItem* items) {
+                                                           const VolItem* items,
+                                                           int undone_only) {
   const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (bv.mode != MCGPU_NB_CELL) return;
+  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c < cells_total(bv.cells)) bv.cells.count[c] = 0;
   if (c == 0) bv.cells.overflow[0] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_cells_assign_multi(const BatchBox* table,
-                                                            const VolItem* items) {
+                                                            const VolItem* items,
+                                                            int undone_only) {
   const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (bv.mode != MCGPU_NB_CELL) return;
+  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
   const int m = blockIdx.x * 256 + threadIdx.x;
   if (m < bv.n_mol) cells_assign_one(bv, m);
 }
 
 __global__ __launch_bounds__(256) void k_cells_sort_multi(const BatchBox* table,
-                                                          const VolItem* items) {
+                                                          const VolItem* items,
+                                                          int undone_only) {
   const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (bv.mode != MCGPU_NB_CELL) return;
+  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
   const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (c < cells_total(bv.cells)) cells_sort_wave(bv.cells, c);
 }
@@ -1033,14 +1042,14 @@ int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 }
 
 int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
-                      int max_cells, hipStream_t st) {
+                      int max_cells, hipStream_t st, int undone_only) {
   if (max_cells == 0) return MCGPU_OK;
   hipLaunchKernelGGL(k_cells_clear_multi, dim3((max_cells + 255) / 256, k), dim3(256), 0, st,
-                     b->d_table, items);
+                     b->d_table, items, undone_only);
   hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0, st,
-                     b->d_table, items);
+                     b->d_table, items, undone_only);
   hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0, st,
-                     b->d_table, items);
+                     b->d_table, items, undone_only);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
@@ -1088,7 +1097,7 @@ int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_
   hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st)) ||
+  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 0)) ||
       (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st, any_fp64, any_mixed)))
     return rc;
   hipLaunchKernelGGL(k_vol_decide, dim3((K + 63) / 64), dim3(64), 0, st, b->d_table, b->d_ctl,
@@ -1096,9 +1105,8 @@ int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_
   hipLaunchKernelGGL(k_restore_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
                      b->d_table, b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  // cells again for the restored boxes (a rebuild of an accepted box's
-  // lists reproduces them: same positions and geometry, ordered slots)
-  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st);
+  // cells again for the restored boxes
+  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 1);
 }
 
 // Work-group-size tuner.  Per box class (neighbour source, pair precision,
@@ -1666,11 +1674,13 @@ namespace {
 
 // Replays G on `st` (the late graph on the volume stream, after the volume
 // pass), capturing it again when the grids, sweep count or time budget
-// changed; e0 and e1 (optional) are recorded around the replay.  Grids are
-// rounded up (surplus work-groups exit at once), so a graph survives most
-// admissions and retirements.  The late graph keeps every group at 32
-// work-groups or more (empty ones exit at once): its box set changes every
-// tick.
+// changed; e0 and e1 (optional) are recorded around the replay.  Every group
+// is launched, with its list's count rounded up to a multiple of 32
+// work-groups (32 at least; surplus work-groups exit at once), and a grid is
+// kept while it is at most twice what the list needs, so a graph survives
+// most admissions, retirements and volume stops: capturing and instantiating
+// one takes the host a few hundred microseconds, during which the device
+// would idle.
 int launch_tick_graph(mcgpu_batch* b, TickGraph& G, bool late, hipStream_t st,
                       const int n[kGroups], int sweeps, hipEvent_t e0, hipEvent_t e1) {
   const int stride = b->max_boxes + 1;
@@ -1678,9 +1688,9 @@ int launch_tick_graph(mcgpu_batch* b, TickGraph& G, bool late, hipStream_t st,
   int grid[kGroups];
   bool same = G.exec && G.version == b->version && G.sweeps == sweeps;
   for (int g = 0; g < kGroups; g++) {
-    const int want = late ? std::max(n[g], 1) : n[g];
-    grid[g] = want ? std::min(b->max_boxes, (want + 31) & ~31) : 0;
-    if (late && G.exec && G.grid[g] >= grid[g]) grid[g] = G.grid[g];
+    const int need = std::min(b->max_boxes, (std::max(n[g], 1) + 31) & ~31);
+    const int cur = G.exec ? G.grid[g] : 0;
+    grid[g] = cur >= need && cur <= 2 * need ? cur : need;
     same = same && G.grid[g] == grid[g];
   }
   if (!same) {
@@ -1694,7 +1704,6 @@ int launch_tick_graph(mcgpu_batch* b, TickGraph& G, bool late, hipStream_t st,
     for (int g = 1; g < kGroups; g++)
       MCGPU_HIP_TRY(hipStreamWaitEvent(b->side[g - 1], b->fork, 0));
     for (int g = 0; g < kGroups; g++) {
-      if (!grid[g]) continue;
       using K = void (*)(const BatchBox*, BoxCtl*, const int32_t*, int, long long,
                          const long long*);
       const K k[2][kGroups] = {{k_batch<256, false>, k_batch<512, false>, k_batch<1024, false>},

@@ -52,20 +52,32 @@ P_ATM = 1.4584e-5  # 1 atm in kcal/mol/A^3
 def npt_against_oracle(hb, steps, vi, delta, neighbor, precision=m.FP64, seed=None,
                        chunk=None, e_rel=1e-9):
     """Runs hb under NPT on the device (record_trace) and the oracle's NPT
-    loop with the same seed: every move and every volume move must agree.
-    Each device volume decision is also replayed through the oracle's
-    acceptance test (SimCalcs::acceptVolumeMove) on the device's own V, V',
-    E, E' and u.  Returns the device's volume-move records."""
+    loop with the same seed: every move and every volume move must agree
+    (npt_matches_oracle).  Returns the device's volume-move records."""
     a = hb.arrays()
     v0 = float(a["box"][0] * a["box"][1] * a["box"][2])
     seed = hb.mc_seed if seed is None else seed
-    ob = O.OracleBox(a)
-    fin, e0, nacc, t, v = ob.run_npt(seed, steps, vi, P_ATM, delta * v0, use_libc=False)
     b = m.Batch(2)
     s = b.add(hb, steps=steps, seed=seed, ensemble=m.NPT, volume_interval=vi,
               pressure=P_ATM, max_volume_delta=delta, neighbor=neighbor,
               precision=precision, record_trace=True)
-    (st,) = run_all(b, [s], steps, chunk or vi)
+    run_all(b, [s], steps, chunk or vi)
+    vg = npt_matches_oracle(b, s, hb, steps, vi, delta * v0, seed, precision, e_rel)
+    b.close()
+    return vg
+
+
+def npt_matches_oracle(b, s, hb, steps, vi, dv, seed, precision=m.FP64, e_rel=1e-9):
+    """Checks a finished NPT box of batch b (added with record_trace) against
+    the oracle's NPT loop with the same seed and volume step dv: moves,
+    volume moves, their energies, and the final running energy.  Each device
+    volume decision is also replayed through the oracle's acceptance test
+    (SimCalcs::acceptVolumeMove) on the device's own V, V', E, E' and u.
+    Returns the device's volume-move records."""
+    a = hb.arrays()
+    ob = O.OracleBox(a)
+    fin, e0, nacc, t, v = ob.run_npt(seed, steps, vi, P_ATM, dv, use_libc=False)
+    st = b.stats(s)
     tr, vg = b.trace(s), b.volume_trace(s)
     assert st.steps_done == steps and st.done and st.fault == 0
     assert st.start_energy == pytest.approx(e0, rel=1e-11 if precision == m.FP64 else e_rel)
@@ -102,5 +114,4 @@ def npt_against_oracle(hb, steps, vi, delta, neighbor, precision=m.FP64, seed=No
     e = fresh.system_energy() + fresh.lrc_species()
     assert st.energy == pytest.approx(e, rel=1e-9 if precision == m.FP64 else 1e-7, abs=1e-6)
     assert st.energy == pytest.approx(fin, rel=1e-9 if precision == m.FP64 else 1e-7, abs=1e-6)
-    b.close()
     return vg

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import mcgpu_amd as m
-from gpu_checks import DE_TOL, de_ok, npt_against_oracle, run_all
+from gpu_checks import P_ATM, DE_TOL, de_ok, npt_against_oracle, npt_matches_oracle, run_all
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -96,6 +96,43 @@ def test_batch_npt_trace_matches_oracle():
     vg = npt_against_oracle(load("mixture300"), 2000, 100, 0.002, m.NEIGHBOR_BRUTE,
                             seed=1)
     assert 0 < vg["accepted"].sum() < len(vg["accepted"])
+
+
+def test_batch_volume_moves_beside_running_boxes_under_a_tick_budget():
+    """Budgeted ticks stop NPT boxes at their volume moves mid-tick; the next
+    tick runs those volume moves beside the other boxes' moves and then moves
+    the NPT boxes until the same deadline.  Every box's moves and volume
+    moves must equal its own oracle run."""
+    steps, vi, delta = 1500, 100, 0.002
+    b = m.Batch(8)
+    b.set_tick_budget(250.0)
+    npt, nvt = [], []
+    for name, seed in (("mixture300", 1), ("water256", 5)):
+        hb = load(name)
+        a = hb.arrays()
+        v0 = float(a["box"][0] * a["box"][1] * a["box"][2])
+        s = b.add(hb, steps=steps, seed=seed, ensemble=m.NPT, volume_interval=vi,
+                  pressure=P_ATM, max_volume_delta=delta, record_trace=True)
+        npt.append((s, hb, seed, delta * v0))
+    for name, seed in (("water256", 7), ("bicyclic200", 3), ("water128_mp", 11)):
+        hb = load(name)
+        nvt.append((b.add(hb, steps=steps, seed=seed, record_trace=True), hb, seed))
+    slots = [x[0] for x in npt + nvt]
+    for _ in range(5000):  # ~40 moves per tick
+        if all(b.stats(s).done for s in slots):
+            break
+        b.run(2000)
+    else:
+        raise AssertionError("boxes did not finish")
+    for s, hb, seed, dv in npt:
+        vg = npt_matches_oracle(b, s, hb, steps, vi, dv, seed)
+        assert 0 < vg["accepted"].sum() < len(vg["accepted"])
+    for s, hb, seed in nvt:
+        _, _, _, t = O.OracleBox(hb.arrays()).run_nvt(seed, steps, use_libc=False)
+        tr = b.trace(s)
+        assert np.array_equal(tr.mol, t["mol"]) and np.array_equal(tr.accepted, t["accepted"])
+        assert de_ok(tr.de, t)
+    b.close()
 
 
 def test_batch_npt_large_molecules_with_cells():
