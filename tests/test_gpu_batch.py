@@ -135,6 +135,36 @@ def test_batch_volume_moves_beside_running_boxes_under_a_tick_budget():
     b.close()
 
 
+def test_batch_remove_npt_boxes_at_their_volume_stop_and_reuse_the_slots():
+    """An NPT box removed while it waits at its volume stop (between ticks)
+    and one removed while the tick in flight runs its volume move and moves
+    after it; new boxes then take the freed slots and memory.  The boxes
+    that stay and the new ones must equal their own oracle runs."""
+    hb = load("water256")
+    npt = dict(ensemble=m.NPT, volume_interval=100, pressure=P_ATM, max_volume_delta=0.002)
+    b = m.Batch(4)
+    a1 = b.add(hb, steps=1000, seed=1, **npt)
+    a2 = b.add(hb, steps=1000, seed=2, **npt)
+    keep = b.add(hb, steps=600, seed=7, record_trace=True)
+    b.run(100)  # both NPT boxes stop at their first volume move
+    assert b.stats(a1).steps_done == 100 and b.stats(a1).volume_moves == 0
+    b.remove(a1)  # nothing of its volume move is queued yet
+    c = b.add(hb, steps=300, seed=3, record_trace=True)
+    b.run(50, sync=False)  # a2 takes its volume move, then moves, in this tick
+    b.remove(a2)  # waits for that tick
+    d = b.add(hb, steps=300, seed=4, record_trace=True)
+    run_all(b, [keep, c, d], 600, 100)
+    for s, seed, steps in ((keep, 7, 600), (c, 3, 300), (d, 4, 300)):
+        fin, _, nacc, t = O.OracleBox(hb.arrays()).run_nvt(seed, steps, use_libc=False)
+        tr = b.trace(s)
+        assert np.array_equal(tr.mol, t["mol"]) and np.array_equal(tr.accepted, t["accepted"])
+        assert de_ok(tr.de, t)
+        st = b.stats(s)
+        assert st.steps_done == steps and st.accepted == nacc
+        assert st.energy == pytest.approx(fin, rel=1e-11, abs=1e-7)
+    b.close()
+
+
 def test_batch_npt_large_molecules_with_cells():
     """NPT on 16-site molecules with 5 cells per edge: the volume moves'
     full energies take the parity pair split over all 27 cells."""
