@@ -1698,20 +1698,20 @@ int launch_tick_graph(mcgpu_batch* b, TickGraph& G, bool late, hipStream_t st,
     if (G.graph) MCGPU_HIP_TRY(hipGraphDestroy(G.graph));
     G.exec = nullptr;
     G.graph = nullptr;
-    hipStream_t on[kGroups] = {st, b->side[0], b->side[1]};
+    using K = void (*)(const BatchBox*, BoxCtl*, const int32_t*, int, long long,
+                       const long long*);
+    const K kern[2][kGroups] = {
+        {k_batch<256, false>, k_batch<512, false>, k_batch<1024, false>},
+        {k_batch<256, true>, k_batch<512, true>, k_batch<1024, true>}};
+    const hipStream_t on[kGroups] = {st, b->side[0], b->side[1]};
     MCGPU_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     MCGPU_HIP_TRY(hipEventRecord(b->fork, st));
     for (int g = 1; g < kGroups; g++)
       MCGPU_HIP_TRY(hipStreamWaitEvent(b->side[g - 1], b->fork, 0));
-    for (int g = 0; g < kGroups; g++) {
-      using K = void (*)(const BatchBox*, BoxCtl*, const int32_t*, int, long long,
-                         const long long*);
-      const K k[2][kGroups] = {{k_batch<256, false>, k_batch<512, false>, k_batch<1024, false>},
-                               {k_batch<256, true>, k_batch<512, true>, k_batch<1024, true>}};
-      hipLaunchKernelGGL(k[late][g], dim3(grid[g]), dim3(kGroupNT[g]), 0, on[g], b->d_table,
-                         b->d_ctl, act + g * stride, sweeps, b->budget_ticks,
+    for (int g = 0; g < kGroups; g++)
+      hipLaunchKernelGGL(kern[late][g], dim3(grid[g]), dim3(kGroupNT[g]), 0, on[g],
+                         b->d_table, b->d_ctl, act + g * stride, sweeps, b->budget_ticks,
                          (const long long*)b->d_t0);
-    }
     for (int g = 1; g < kGroups; g++) {
       MCGPU_HIP_TRY(hipEventRecord(b->join[g - 1], b->side[g - 1]));
       MCGPU_HIP_TRY(hipStreamWaitEvent(st, b->join[g - 1], 0));
