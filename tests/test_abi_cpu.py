@@ -34,6 +34,25 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+@pytest.mark.parametrize("given,want", [(None, "16"), ("4", "16"), ("32", "32")])
+def test_loading_the_library_raises_hardware_queues(given, want):
+    """The package asks HIP for at least 16 hardware queues before the
+    library starts HIP, and keeps a higher setting (fresh process each)."""
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ)
+    env.pop("GPU_MAX_HW_QUEUES", None)
+    if given is not None:
+        env["GPU_MAX_HW_QUEUES"] = given
+    code = ("import os, mcgpu_amd as m; m.lib(); "
+            "print(os.environ['GPU_MAX_HW_QUEUES'])")
+    out = subprocess.run([sys.executable, "-c", code], env=env, cwd=ROOT,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split()[-1] == want
+
+
 def test_no_device_here_fails_loudly():
     if m.device_count() > 0:
         pytest.skip("a GPU is visible")
