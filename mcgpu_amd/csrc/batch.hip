@@ -533,18 +533,13 @@ __global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
 // Second half, after the full energy at the new volume (out[2 i + 1]):
 // the acceptance test, the running energy and LRC on acceptance, the old
 // edges and geometry (and the undo flag for the coordinates) on rejection,
-// and the box's next volume stop.
-__global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
-                             const double* out) {
+// and the box's next volume stop.  e_pairs: the pair energy at the new
+// volume.
+__device__ void vol_decide(BatchBox& B, BoxCtl& c, VolItem& item, double e_pairs) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k) return;
-  const int s = items[i].slot;
-  BatchBox& B = table[s];
-  BoxCtl& c = ctl[s];
   const bool valid = c.v_new > 0.0;
   const double lrc_new = valid ? c.lrc * c.v_old / c.v_new : c.lrc;
-  const double e_new = valid ? out[2 * i + 1] + lrc_new : c.energy;
+  const double e_new = valid ? e_pairs + lrc_new : c.energy;
   bool accept = false;
   if (valid) {
     // SimCalcs::acceptVolumeMove (SimulationStep.cpp:476-488)
@@ -575,7 +570,7 @@ __global__ void k_vol_decide(BatchBox* table, BoxCtl* ctl, VolItem* items, int k
   } else {
     for (int d = 0; d < 3; d++) B.bv.L[d] = c.L_old[d];
     set_geometry_dev(B);
-    items[i].undo = 1;
+    item.undo = 1;
   }
   c.next_stop = c.steps_target < c.steps_done + B.vol_interval ? c.steps_target
                                                                : c.steps_done + B.vol_interval;
@@ -595,6 +590,24 @@ __global__ __launch_bounds__(256) void k_energy_waves(const BatchBox* table,
   full_energy_block<MIXED>(B.bv, B.partial, nullptr, ws, p64, p32);
 }
 
+// Entry blockIdx.x of a volume pass: the fixed-order sum of its
+// per-molecule partials at the new volume (as k_reduce_multi sums them),
+// then the decision.
+__global__ __launch_bounds__(256) void k_vol_settle(BatchBox* table, BoxCtl* ctl,
+                                                    VolItem* items) {
+  __shared__ double red[4];
+  VolItem& item = items[blockIdx.x];
+  BatchBox& B = table[item.slot];
+  const int n = B.bv.n_mol;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += B.partial[i];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) {
+    B.total[1] = s;
+    vol_decide(B, ctl[item.slot], item, s);
+  }
+}
+
 // Fixed-order sum of entry blockIdx.x's per-molecule partials into
 // total[which] and out[2 * entry + which].
 __global__ __launch_bounds__(256) void k_reduce_multi(const BatchBox* table,
@@ -612,18 +625,69 @@ __global__ __launch_bounds__(256) void k_reduce_multi(const BatchBox* table,
   }
 }
 
+// Clears entry y's cell occupancies for its current geometry (block x's
+// 256 cells), when the box is in cell mode.
+__device__ __forceinline__ void clear_cells_block(const BoxView& bv) {
+  if (bv.mode != MCGPU_NB_CELL) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < cells_total(bv.cells)) bv.cells.count[c] = 0;
+  if (c == 0) bv.cells.overflow[0] = 0;
+}
+
 // NPT rescale of entry y: molecule m is translated so its first primary
 // moves from p to f p (internal geometry untouched); its sites are backed up
-// first and its primary mirror entries refreshed.
+// first and its primary mirror entries refreshed.  The entry's cell
+// occupancies are cleared for the new geometry on the way (block x clears
+// cells [256 x, 256 x + 256)).  A uniform box's block takes whole molecules
+// (256 / len of them), one lane per site, so its loads and stores are
+// contiguous: the first primaries' shifts go through LDS before any site is
+// written.  Other boxes take one molecule per lane.
 __global__ __launch_bounds__(256) void k_scale_multi(const BatchBox* table,
                                                      const VolItem* items) {
+  __shared__ double shift[3][256];
   const VolItem it = items[blockIdx.y];
   const BatchBox& B = table[it.slot];
   const BoxView& bv = B.bv;
-  const int m = blockIdx.x * 256 + threadIdx.x;
+  clear_cells_block(bv);
+  const double f = it.f;
+  const int t = threadIdx.x;
+  const int len = bv.uni_len;
+  if (len > 0 && bv.single_primary) {
+    const int per = 256 / len;  // molecules per block
+    const int m0 = blockIdx.x * per;
+    if (m0 >= bv.n_mol) return;
+    const int nm = bv.n_mol - m0 < per ? bv.n_mol - m0 : per;
+    if (t < nm) {
+      const int m = m0 + t;
+      const int p = m * len + bv.uni_p0;
+      const double px = bv.x[p], py = bv.y[p], pz = bv.z[p];
+      const double sx = px * f - px;
+      const double sy = py * f - py;
+      const double sz = pz * f - pz;
+      shift[0][t] = sx;
+      shift[1][t] = sy;
+      shift[2][t] = sz;
+      bv.px[m] = px + sx;
+      bv.py[m] = py + sy;
+      bv.pz[m] = pz + sz;
+    }
+    __syncthreads();
+    if (t < nm * len) {
+      const int s = m0 * len + t;
+      const int k = t / len;
+      const double x = bv.x[s], y = bv.y[s], z = bv.z[s];
+      B.bx[s] = x;
+      B.by[s] = y;
+      B.bz[s] = z;
+      bv.x[s] = x + shift[0][k];
+      bv.y[s] = y + shift[1][k];
+      bv.z[s] = z + shift[2][k];
+    }
+    return;
+  }
+  const int m = blockIdx.x * 256 + t;
   if (m >= bv.n_mol) return;
   const int p = bv.pidx[bv.mol_pidx_start[m]];
-  const double f = it.f;
   const double sx = bv.x[p] * f - bv.x[p];
   const double sy = bv.y[p] * f - bv.y[p];
   const double sz = bv.z[p] * f - bv.z[p];
@@ -645,13 +709,38 @@ __global__ __launch_bounds__(256) void k_scale_multi(const BatchBox* table,
   }
 }
 
-// A rejected rescale undone: molecule m's sites and primaries restored.
+// A rejected rescale undone: molecule m's sites and primaries restored, and
+// the cell occupancies cleared for the restored geometry.  Blocks map to
+// molecules as in k_scale_multi.
 __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
                                                        const VolItem* items) {
   if (!items[blockIdx.y].undo) return;
   const BatchBox& B = table[items[blockIdx.y].slot];
   const BoxView& bv = B.bv;
-  const int m = blockIdx.x * 256 + threadIdx.x;
+  clear_cells_block(bv);
+  const int t = threadIdx.x;
+  const int len = bv.uni_len;
+  if (len > 0 && bv.single_primary) {
+    const int per = 256 / len;
+    const int m0 = blockIdx.x * per;
+    if (m0 >= bv.n_mol) return;
+    const int nm = bv.n_mol - m0 < per ? bv.n_mol - m0 : per;
+    if (t < nm * len) {
+      const int s = m0 * len + t;
+      bv.x[s] = B.bx[s];
+      bv.y[s] = B.by[s];
+      bv.z[s] = B.bz[s];
+    }
+    if (t < nm) {
+      const int m = m0 + t;
+      const int p = m * len + bv.uni_p0;
+      bv.px[m] = B.bx[p];
+      bv.py[m] = B.by[p];
+      bv.pz[m] = B.bz[p];
+    }
+    return;
+  }
+  const int m = blockIdx.x * 256 + t;
   if (m >= bv.n_mol) return;
   const int s0 = bv.mol_start[m], s1 = s0 + bv.mol_len[m];
   for (int s = s0; s < s1; s++) {
@@ -669,21 +758,12 @@ __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
 }
 
 // Cell lists of every listed box in cell mode (with undone_only, of the
-// boxes whose volume move was undone), rebuilt for its current geometry:
-// clear, assign, order (see cells.h).
+// boxes whose volume move was undone), rebuilt for its current geometry
+// after k_scale_multi or k_restore_multi cleared the occupancies: assign,
+// order (see cells.h).
 __device__ __forceinline__ bool cells_skip(const BoxView& bv, const VolItem& it,
                                            int undone_only) {
   return bv.mode != MCGPU_NB_CELL || (undone_only && !it.undo);
-}
-
-__global__ __launch_bounds__(256) void k_cells_clear_multi(const BatchBox* table,
-                                                           const VolItem* items,
-                                                           int undone_only) {
-  const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < cells_total(bv.cells)) bv.cells.count[c] = 0;
-  if (c == 0) bv.cells.overflow[0] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_cells_assign_multi(const BatchBox* table,
@@ -948,7 +1028,6 @@ struct mcgpu_batch {
   VolItem* d_items = nullptr;   // [max_boxes] list of a volume pass
   VolItem* h_items = nullptr;   // pinned
   VolItem* d_one = nullptr;     // [max_boxes] entry {s} of slot s (admission)
-  double* d_volout = nullptr;   // [2 * max_boxes] energies of a volume pass
   long long* d_t0 = nullptr;    // wall clock at the start of the volume pass
   long long budget_ticks = 0;   // per-launch time limit (wall-clock ticks)
   double clock_khz = 100000.0;
@@ -1026,26 +1105,21 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
   return n[0] * n[1] * n[2];
 }
 
-// The batched passes over the first k entries of a device list: full
-// energies into d_volout[2 i + which]; cell lists for the current geometry.
-int launch_energy_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
-                       int which, hipStream_t st, bool any_fp64, bool any_mixed) {
+// The batched passes over the first k entries of a device list: per-molecule
+// full-energy partials; cell lists for the current geometry (occupancies
+// already cleared).
+void launch_energy_partials(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
+                            hipStream_t st, bool any_fp64, bool any_mixed) {
   const dim3 grid((max_mol + 3) / 4, k);  // blocks past a box's molecules exit at once
   if (any_fp64)
     hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
   if (any_mixed)
     hipLaunchKernelGGL(k_energy_waves<true>, grid, dim3(256), 0, st, b->d_table, items);
-  hipLaunchKernelGGL(k_reduce_multi, dim3(k), dim3(256), 0, st, b->d_table, items, which,
-                     b->d_volout);
-  MCGPU_HIP_TRY(hipGetLastError());
-  return MCGPU_OK;
 }
 
 int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
                       int max_cells, hipStream_t st, int undone_only) {
   if (max_cells == 0) return MCGPU_OK;
-  hipLaunchKernelGGL(k_cells_clear_multi, dim3((max_cells + 255) / 256, k), dim3(256), 0, st,
-                     b->d_table, items, undone_only);
   hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0, st,
                      b->d_table, items, undone_only);
   hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0, st,
@@ -1087,23 +1161,30 @@ int begin_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t
 int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
   int rc;
   const int K = (int)pend.size();
-  int max_mol = 0, max_cells = 0;
+  int max_mol = 0, max_cells = 0, scale_blocks = 0;
   bool any_fp64 = false, any_mixed = false;
   for (int s : pend) {
+    const BoxView& v = b->h_table[s].bv;
     (b->slots[s].params.precision == MCGPU_MIXED ? any_mixed : any_fp64) = true;
-    max_mol = std::max(max_mol, b->h_table[s].bv.n_mol);
+    max_mol = std::max(max_mol, v.n_mol);
     max_cells = std::max(max_cells, b->slots[s].lay.ncell);
+    // k_scale_multi and k_restore_multi: whole molecules per block for
+    // uniform boxes
+    const int per = v.uni_len > 0 && v.single_primary ? 256 / v.uni_len : 256;
+    scale_blocks = std::max(scale_blocks, (v.n_mol + per - 1) / per);
   }
-  hipLaunchKernelGGL(k_scale_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
-                     b->d_table, b->d_items);
+  // blocks for the occupancies too (at most the allocated cells)
+  const int clear_blocks = (max_cells + 255) / 256;
+  const int rescale_grid = std::max(scale_blocks, clear_blocks);
+  hipLaunchKernelGGL(k_scale_multi, dim3(rescale_grid, K), dim3(256), 0, st, b->d_table,
+                     b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 0)) ||
-      (rc = launch_energy_pass(b, b->d_items, K, max_mol, 1, st, any_fp64, any_mixed)))
-    return rc;
-  hipLaunchKernelGGL(k_vol_decide, dim3((K + 63) / 64), dim3(64), 0, st, b->d_table, b->d_ctl,
-                     b->d_items, K, (const double*)b->d_volout);
-  hipLaunchKernelGGL(k_restore_multi, dim3((max_mol + 255) / 256, K), dim3(256), 0, st,
-                     b->d_table, b->d_items);
+  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 0))) return rc;
+  launch_energy_partials(b, b->d_items, K, max_mol, st, any_fp64, any_mixed);
+  hipLaunchKernelGGL(k_vol_settle, dim3(K), dim3(256), 0, st, b->d_table, b->d_ctl,
+                     b->d_items);
+  hipLaunchKernelGGL(k_restore_multi, dim3(rescale_grid, K), dim3(256), 0, st, b->d_table,
+                     b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
   // cells again for the restored boxes
   return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 1);
@@ -1291,7 +1372,6 @@ extern "C" int mcgpu_batch_create(int32_t max_boxes, size_t budget,
        hipMalloc(&b->d_active, sizeof(int32_t) * 2 * kGroups * (M + 1)) == hipSuccess &&
        hipMalloc(&b->d_items, sizeof(VolItem) * M) == hipSuccess &&
        hipMalloc(&b->d_one, sizeof(VolItem) * M) == hipSuccess &&
-       hipMalloc(&b->d_volout, sizeof(double) * 2 * M) == hipSuccess &&
        hipMalloc(&b->d_t0, sizeof(long long)) == hipSuccess &&
        hipHostMalloc((void**)&b->h_table, sizeof(BatchBox) * M, 0) == hipSuccess &&
        hipHostMalloc((void**)&b->h_ctl, sizeof(BoxCtl) * M, 0) == hipSuccess &&
@@ -1340,7 +1420,7 @@ extern "C" int mcgpu_batch_destroy(mcgpu_batch* b) {
     if (G->graph) (void)hipGraphDestroy(G->graph);
   }
   for (void* p : {(void*)b->d_table, (void*)b->d_ctl, (void*)b->d_rng, (void*)b->d_active,
-                  (void*)b->d_items, (void*)b->d_one, (void*)b->d_volout, (void*)b->d_t0})
+                  (void*)b->d_items, (void*)b->d_one, (void*)b->d_t0})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->h_table, (void*)b->h_ctl, (void*)b->h_ctl_rb, (void*)b->h_rng,
                   (void*)b->h_rng_rb, (void*)b->h_active, (void*)b->h_items})
