@@ -84,6 +84,12 @@ struct BatchBox {
   int32_t vol_interval;
   int32_t ncell_cap;
   int64_t tr_vol_cap;
+  // NPT cell boxes: the other set of cell-list arrays and its mode.  A
+  // volume move builds the new geometry's lists in it and swaps it in; an
+  // undone move swaps the untouched lists back.
+  CellGrid cells_alt;
+  int32_t mode_alt;
+  int32_t pad_alt;
 };
 
 // One box of a batched full-energy / rescale / cell pass.
@@ -527,6 +533,12 @@ __global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
   items[i].undo = 0;
   for (int d = 0; d < 3; d++) v.L[d] = c.L_old[d] * f;
   c.v_new = v_prop > 0.0 ? v.L[0] * v.L[1] * v.L[2] : v_prop;
+  // the new geometry's cell lists go to the other set of arrays; the
+  // current lists stay as they are for an undo
+  const CellGrid cur = v.cells;
+  v.cells = B.cells_alt;
+  B.cells_alt = cur;
+  B.mode_alt = v.mode;
   set_geometry_dev(B);
 }
 
@@ -568,8 +580,12 @@ __device__ void vol_decide(BatchBox& B, BoxCtl& c, VolItem& item, double e_pairs
     c.lrc = lrc_new;
     for (int d = 0; d < 3; d++) c.L[d] = B.bv.L[d];
   } else {
+    // the old edges, and the cell lists and mode from before the move
     for (int d = 0; d < 3; d++) B.bv.L[d] = c.L_old[d];
-    set_geometry_dev(B);
+    const CellGrid cur = B.bv.cells;
+    B.bv.cells = B.cells_alt;
+    B.cells_alt = cur;
+    B.bv.mode = B.mode_alt;
     item.undo = 1;
   }
   c.next_stop = c.steps_target < c.steps_done + B.vol_interval ? c.steps_target
@@ -577,8 +593,8 @@ __device__ void vol_decide(BatchBox& B, BoxCtl& c, VolItem& item, double e_pairs
   c.status = c.steps_done >= c.steps_target ? kDone : kRunning;
 }
 
-// One instance per precision: each takes the listed boxes of its precision
-// (the other instance's blocks return at once).
+// One instance per precision (admissions: the start energy of one box);
+// blocks whose entry has the other precision return at once.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_energy_waves(const BatchBox* table,
                                                       const VolItem* items) {
@@ -606,6 +622,20 @@ __global__ __launch_bounds__(256) void k_vol_settle(BatchBox* table, BoxCtl* ctl
     B.total[1] = s;
     vol_decide(B, ctl[item.slot], item, s);
   }
+}
+
+// Both precisions in one launch (a volume pass lists boxes of either): each
+// block takes its entry's.
+__global__ __launch_bounds__(256) void k_energy_waves_any(const BatchBox* table,
+                                                          const VolItem* items) {
+  __shared__ WaveEval ws[4];
+  __shared__ SiteParam64 p64[kParamCap];
+  __shared__ SiteParam32 p32[kParamCap];
+  const BatchBox& B = table[items[blockIdx.y].slot];
+  if (B.precision == MCGPU_MIXED)
+    full_energy_block<true>(B.bv, B.partial, nullptr, ws, p64, p32);
+  else
+    full_energy_block<false>(B.bv, B.partial, nullptr, ws, p64, p32);
 }
 
 // Fixed-order sum of entry blockIdx.x's per-molecule partials into
@@ -709,15 +739,14 @@ __global__ __launch_bounds__(256) void k_scale_multi(const BatchBox* table,
   }
 }
 
-// A rejected rescale undone: molecule m's sites and primaries restored, and
-// the cell occupancies cleared for the restored geometry.  Blocks map to
+// A rejected rescale undone: molecule m's sites and primaries restored (the
+// decision put the cell lists from before the move back).  Blocks map to
 // molecules as in k_scale_multi.
 __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
                                                        const VolItem* items) {
   if (!items[blockIdx.y].undo) return;
   const BatchBox& B = table[items[blockIdx.y].slot];
   const BoxView& bv = B.bv;
-  clear_cells_block(bv);
   const int t = threadIdx.x;
   const int len = bv.uni_len;
   if (len > 0 && bv.single_primary) {
@@ -757,29 +786,20 @@ __global__ __launch_bounds__(256) void k_restore_multi(const BatchBox* table,
   }
 }
 
-// Cell lists of every listed box in cell mode (with undone_only, of the
-// boxes whose volume move was undone), rebuilt for its current geometry
-// after k_scale_multi or k_restore_multi cleared the occupancies: assign,
-// order (see cells.h).
-__device__ __forceinline__ bool cells_skip(const BoxView& bv, const VolItem& it,
-                                           int undone_only) {
-  return bv.mode != MCGPU_NB_CELL || (undone_only && !it.undo);
-}
-
+// Cell lists of every listed box in cell mode, built for the new geometry
+// after k_scale_multi cleared the occupancies: assign, order (see cells.h).
 __global__ __launch_bounds__(256) void k_cells_assign_multi(const BatchBox* table,
-                                                            const VolItem* items,
-                                                            int undone_only) {
+                                                            const VolItem* items) {
   const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
+  if (bv.mode != MCGPU_NB_CELL) return;
   const int m = blockIdx.x * 256 + threadIdx.x;
   if (m < bv.n_mol) cells_assign_one(bv, m);
 }
 
 __global__ __launch_bounds__(256) void k_cells_sort_multi(const BatchBox* table,
-                                                          const VolItem* items,
-                                                          int undone_only) {
+                                                          const VolItem* items) {
   const BoxView& bv = table[items[blockIdx.y].slot].bv;
-  if (cells_skip(bv, items[blockIdx.y], undone_only)) return;
+  if (bv.mode != MCGPU_NB_CELL) return;
   const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (c < cells_total(bv.cells)) cells_sort_wave(bv.cells, c);
 }
@@ -852,6 +872,8 @@ struct BoxLayout {
   size_t x, y, z, px, py, pz, ms, ml, mps, mpc, mp, meta, pidx, p64, p32;
   size_t ccount, cslots, mcell, mslot, covf, vstale, vcount, partial, pairs,
       total, bx, by, bz, trm, trd, tra, trv;
+  size_t ccount2, cslots2, mcell2, mslot2, covf2;  // NPT cell boxes: second set
+  bool alt_cells;
   int64_t trv_cap;
   size_t bytes;
   int ncell, cap;
@@ -900,6 +922,14 @@ BoxLayout plan_layout(const mcgpu_box_desc& d, const mcgpu_run_params& p,
   L.mcell = take(4 * nm);
   L.mslot = take(4 * nm);
   L.covf = take(4);
+  L.alt_cells = L.ncell > 0 && p.ensemble == MCGPU_NPT;
+  if (L.alt_cells) {
+    L.ccount2 = take(4 * (size_t)L.ncell);
+    L.cslots2 = take(4 * (size_t)L.ncell * L.cap);
+    L.mcell2 = take(4 * nm);
+    L.mslot2 = take(4 * nm);
+    L.covf2 = take(4);
+  }
   L.vstale = take(4);
   L.vcount = take(4);
   L.partial = take(8 * std::max(ns, nm));  // per-site partial energies
@@ -1109,21 +1139,18 @@ int slot_set_geometry(mcgpu_batch* b, int s) {
 // full-energy partials; cell lists for the current geometry (occupancies
 // already cleared).
 void launch_energy_partials(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
-                            hipStream_t st, bool any_fp64, bool any_mixed) {
+                            hipStream_t st) {
   const dim3 grid((max_mol + 3) / 4, k);  // blocks past a box's molecules exit at once
-  if (any_fp64)
-    hipLaunchKernelGGL(k_energy_waves<false>, grid, dim3(256), 0, st, b->d_table, items);
-  if (any_mixed)
-    hipLaunchKernelGGL(k_energy_waves<true>, grid, dim3(256), 0, st, b->d_table, items);
+  hipLaunchKernelGGL(k_energy_waves_any, grid, dim3(256), 0, st, b->d_table, items);
 }
 
 int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
-                      int max_cells, hipStream_t st, int undone_only) {
+                      int max_cells, hipStream_t st) {
   if (max_cells == 0) return MCGPU_OK;
   hipLaunchKernelGGL(k_cells_assign_multi, dim3((max_mol + 255) / 256, k), dim3(256), 0, st,
-                     b->d_table, items, undone_only);
+                     b->d_table, items);
   hipLaunchKernelGGL(k_cells_sort_multi, dim3((max_cells + 3) / 4, k), dim3(256), 0, st,
-                     b->d_table, items, undone_only);
+                     b->d_table, items);
   MCGPU_HIP_TRY(hipGetLastError());
   return MCGPU_OK;
 }
@@ -1131,10 +1158,12 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 // NPT volume moves of the boxes in `pend`, batched and decided on the
 // device, queued on `st` at the start of the tick in which those boxes move
 // on (beside the tick's other boxes): u1, u2 from each box's generator, the
-// rescale (with backup), cells for the new geometry, the full energy, the
-// decision, and the undo of rejected moves.  Semantics (the build's own;
-// the reference's NPT path, Ensemble.cpp:9-28 / SimulationStep.cpp:476-518,
-// always grows the box by 1.1 and never recomputes the energy):
+// rescale (with backup), cells for the new geometry (in the box's second set
+// of cell arrays, so that an undo only swaps the old lists back), the full
+// energy, the decision, and the undo of rejected moves.  Semantics (the
+// build's own; the reference's NPT path, Ensemble.cpp:9-28 /
+// SimulationStep.cpp:476-518, always grows the box by 1.1 and never
+// recomputes the energy):
 //   V' = V + dv (2 u1 - 1), dv = delta V0 (V0 the volume at admission): a
 //   fixed-width, symmetric step, so acceptVolumeMove's test is the whole
 //   acceptance rule; molecules translated so first primaries scale by
@@ -1162,10 +1191,8 @@ int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_
   int rc;
   const int K = (int)pend.size();
   int max_mol = 0, max_cells = 0, scale_blocks = 0;
-  bool any_fp64 = false, any_mixed = false;
   for (int s : pend) {
     const BoxView& v = b->h_table[s].bv;
-    (b->slots[s].params.precision == MCGPU_MIXED ? any_mixed : any_fp64) = true;
     max_mol = std::max(max_mol, v.n_mol);
     max_cells = std::max(max_cells, b->slots[s].lay.ncell);
     // k_scale_multi and k_restore_multi: whole molecules per block for
@@ -1179,15 +1206,14 @@ int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_
   hipLaunchKernelGGL(k_scale_multi, dim3(rescale_grid, K), dim3(256), 0, st, b->d_table,
                      b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 0))) return rc;
-  launch_energy_partials(b, b->d_items, K, max_mol, st, any_fp64, any_mixed);
+  if ((rc = launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st))) return rc;
+  launch_energy_partials(b, b->d_items, K, max_mol, st);
   hipLaunchKernelGGL(k_vol_settle, dim3(K), dim3(256), 0, st, b->d_table, b->d_ctl,
                      b->d_items);
-  hipLaunchKernelGGL(k_restore_multi, dim3(rescale_grid, K), dim3(256), 0, st, b->d_table,
+  hipLaunchKernelGGL(k_restore_multi, dim3(scale_blocks, K), dim3(256), 0, st, b->d_table,
                      b->d_items);
   MCGPU_HIP_TRY(hipGetLastError());
-  // cells again for the restored boxes
-  return launch_cells_pass(b, b->d_items, K, max_mol, max_cells, st, 1);
+  return MCGPU_OK;
 }
 
 // Work-group-size tuner.  Per box class (neighbour source, pair precision,
@@ -1571,6 +1597,15 @@ static int batch_add_impl(mcgpu_batch* b, const mcgpu_box_desc* d, const mcgpu_r
   v.cells.mol_slot = gmem((int32_t*)(base + L.mslot));
   v.cells.overflow = gmem((int32_t*)(base + L.covf));
   v.cells.cap = L.cap;
+  E.cells_alt = v.cells;
+  if (L.alt_cells) {
+    E.cells_alt.count = gmem((int32_t*)(base + L.ccount2));
+    E.cells_alt.slots = gmem((int32_t*)(base + L.cslots2));
+    E.cells_alt.mol_cell = gmem((int32_t*)(base + L.mcell2));
+    E.cells_alt.mol_slot = gmem((int32_t*)(base + L.mslot2));
+    E.cells_alt.overflow = gmem((int32_t*)(base + L.covf2));
+  }
+  E.mode_alt = MCGPU_NB_BRUTE;
   v.verlet.stale = gmem((int32_t*)(base + L.vstale));
   v.verlet.count = gmem((int32_t*)(base + L.vcount));
   E.rng = b->d_rng + s;
@@ -1901,12 +1936,14 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
                                b->ev1))))
     return rc;
   if (any_late) {
+    // the late lists go up while the volume pass waits for the host to
+    // queue its second part
+    if ((rc = upload_lists(b, true, b->vstream))) return rc;
     const double t1 = host_now();
     if ((rc = finish_volume_moves(b, b->late, b->vstream))) return rc;
     tv = host_now() - t1;
     b->t_volume += tv;
-    if ((rc = upload_lists(b, true, b->vstream)) ||
-        (rc = launch_tick_graph(b, b->late_graph, true, b->vstream, n[1], sweeps, nullptr,
+    if ((rc = launch_tick_graph(b, b->late_graph, true, b->vstream, n[1], sweeps, nullptr,
                                 nullptr)))
       return rc;
     MCGPU_HIP_TRY(hipEventRecord(b->ev_vdone, b->vstream));
