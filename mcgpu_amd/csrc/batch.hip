@@ -10,7 +10,7 @@
 // Box memory is carved from one HBM arena (admission by footprint).  NPT
 // volume moves run on a stream of their own beside a launch, decided on the
 // device by a short sequence of multi-box kernels around the full-energy
-// pass (see begin_volume_moves).
+// pass (see queue_volume_moves).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -375,8 +375,8 @@ __device__ __forceinline__ void finish_move(BatchShared<NT>& S, const BatchBox& 
 // decides and commits each move; all waves scan candidates and sum pairs.
 // The LATE instance runs the boxes of a tick whose NPT volume move comes
 // first (the volume pass queued just before it on its stream, beside the
-// rest of the tick): their budget counts from *t0, the wall clock the pass
-// stamped when it began, so they stop with the rest of the tick.
+// rest of the tick): their budget counts from *t0, the wall clock stamped on
+// that stream as the tick began, so they stop with the rest of the tick.
 template <int NT, bool LATE>
 __global__ __launch_bounds__(NT) void k_batch(const BatchBox* __restrict__ table,
                                               BoxCtl* ctl,
@@ -504,18 +504,19 @@ __device__ void set_geometry_dev(BatchBox& B) {
   v.mode = MCGPU_NB_CELL;
 }
 
+// The wall clock into *t0: the start of a tick for the boxes that move after
+// their volume move.
+__global__ void k_stamp_clock(long long* t0) { *t0 = wall_clock64(); }
+
 // NPT volume move, first half, one thread per listed box (semantics: see
-// begin_volume_moves): u1 and u2 drawn from the box's own generator, the
+// queue_volume_moves): u1 and u2 drawn from the box's own generator, the
 // new edges and cell geometry set, the scale factor handed to the rescale.
 // V' is the product of the new edges, as acceptVolumeMove reads it
 // (SimulationStep.cpp:476-481); a proposal <= 0 keeps the box (f = 1) and
-// is rejected by the decision.  Thread 0 stamps the wall clock into *t0: the
-// start of the tick for the boxes that move after their volume move.
-__global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k,
-                            long long* t0) {
+// is rejected by the decision.
+__global__ void k_vol_begin(BatchBox* table, BoxCtl* ctl, VolItem* items, int k) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *t0 = wall_clock64();
   if (i >= k) return;
   const int s = items[i].slot;
   BatchBox& B = table[s];
@@ -1173,23 +1174,14 @@ int launch_cells_pass(mcgpu_batch* b, const VolItem* items, int k, int max_mol,
 //   exp(N ln(V'/V) - (dE + P (V' - V)) / kT) >= u2 (SimulationStep.cpp:
 //   476-488), and then the running energy becomes E' exactly.  A proposal
 //   V' <= 0 is rejected.
-// Queued in two parts, so that the rest of the tick can be launched between
-// them: the first draws u1, u2 and stamps the clock (the late boxes' tick
-// start), the second rescales, evaluates, decides and undoes.
-int begin_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
+int queue_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
+  int rc;
   const int K = (int)pend.size();
   for (int k = 0; k < K; k++) b->h_items[k] = VolItem{pend[k], 0, 1.0};
   MCGPU_HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items, sizeof(VolItem) * K,
                                hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_vol_begin, dim3((K + 63) / 64), dim3(64), 0, st, b->d_table, b->d_ctl,
-                     b->d_items, K, b->d_t0);
-  MCGPU_HIP_TRY(hipGetLastError());
-  return MCGPU_OK;
-}
-
-int finish_volume_moves(mcgpu_batch* b, const std::vector<int>& pend, hipStream_t st) {
-  int rc;
-  const int K = (int)pend.size();
+                     b->d_items, K);
   int max_mol = 0, max_cells = 0, scale_blocks = 0;
   for (int s : pend) {
     const BoxView& v = b->h_table[s].bv;
@@ -1922,12 +1914,13 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
   const bool any_main = b->late.size() < b->launched.size();
   const bool any_late = !b->late.empty();
   double tv = 0.0;
-  // Order: the volume branch's first kernel (its clock stamp), the main
-  // graph, then the rest of the branch, so that both sides start together
-  // and neither waits for the host's queueing of the other.
+  // Order: the volume branch's clock stamp (the late boxes' tick start), the
+  // main graph, then the rest of the branch, so that both sides start
+  // together and the main graph waits for as little host work as possible.
   if (any_late) {
     MCGPU_HIP_TRY(hipStreamWaitEvent(b->vstream, b->ev_ready, 0));
-    if ((rc = begin_volume_moves(b, b->late, b->vstream))) return rc;
+    hipLaunchKernelGGL(k_stamp_clock, dim3(1), dim3(1), 0, b->vstream, b->d_t0);
+    MCGPU_HIP_TRY(hipGetLastError());
   }
   MCGPU_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_ready, 0));
   if (any_main &&
@@ -1936,11 +1929,9 @@ extern "C" int mcgpu_batch_run(mcgpu_batch* b, int32_t sweeps, int32_t sync) {
                                b->ev1))))
     return rc;
   if (any_late) {
-    // the late lists go up while the volume pass waits for the host to
-    // queue its second part
     if ((rc = upload_lists(b, true, b->vstream))) return rc;
     const double t1 = host_now();
-    if ((rc = finish_volume_moves(b, b->late, b->vstream))) return rc;
+    if ((rc = queue_volume_moves(b, b->late, b->vstream))) return rc;
     tv = host_now() - t1;
     b->t_volume += tv;
     if ((rc = launch_tick_graph(b, b->late_graph, true, b->vstream, n[1], sweeps, nullptr,
