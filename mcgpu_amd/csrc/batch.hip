@@ -450,7 +450,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchBox* __restrict__ table
   }
 }
 
-// Start energy of a newly added box: pairs (reduced by k_reduce2 into
+// Start energy of a newly added box: pairs (reduced by k_reduce_multi into
 // total[0]) plus the offset (the LRC), or, for a box that continues from a
 // resume record, that record's running energy.
 __global__ void k_set_start_energy(BoxCtl* c, const double* total, double offset,
@@ -458,18 +458,6 @@ __global__ void k_set_start_energy(BoxCtl* c, const double* total, double offset
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     c->energy = resumed ? energy : total[0] + offset;
     c->start_energy = c->energy;
-  }
-}
-
-// Copies the backup coordinates back after a rejected volume move.
-__global__ void k_restore_coords(double* x, double* y, double* z,
-                                 const double* bx, const double* by,
-                                 const double* bz, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    x[i] = bx[i];
-    y[i] = by[i];
-    z[i] = bz[i];
   }
 }
 
