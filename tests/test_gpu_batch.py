@@ -165,6 +165,22 @@ def test_batch_remove_npt_boxes_at_their_volume_stop_and_reuse_the_slots():
     b.close()
 
 
+def test_batch_npt_volume_moves_that_change_the_cell_grid():
+    """1,500 waters in a 35.6 A box hold 3 cells of 9 A per edge; the box,
+    generated from a lattice, expands under volume steps of up to 10% of V0
+    and passes 36 A, where 4 cells fit: volume moves build their lists on a
+    grid of another size and undone ones swap the previous grid and lists
+    back.  Moves and volume moves must equal the oracle's."""
+    hb = m.HostBox.generate(m.WATER, 1500, edge=35.6, cutoff=9.0, temperature=298.0,
+                            max_translate=0.15, max_rotate=15.0, seed=1500, mc_seed=5)
+    assert int(35.6 // 9.0) == 3
+    vg = npt_against_oracle(hb, 500, 25, 0.1, m.NEIGHBOR_CELL)
+    edges = np.cbrt(vg["v_old"])  # the box's edge at each volume move
+    print(f"volume moves: {len(vg['accepted'])}, accepted {int(vg['accepted'].sum())}, "
+          f"edges {edges.min():.2f}-{edges.max():.2f} A")
+    assert edges.min() < 36.0 < edges.max()
+
+
 def test_batch_npt_large_molecules_with_cells():
     """NPT on 16-site molecules with 5 cells per edge: the volume moves'
     full energies take the parity pair split over all 27 cells."""
