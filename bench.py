@@ -251,12 +251,12 @@ def cpu_model() -> str:
 
 # VALU issue slots per site pair in the full-energy kernel's inner loop
 # (FP64 path without the per-pair wrap, read from this build's gfx950 code
-# object: 29 full-rate VALU instructions -- displacement 3, r^2 3, the rsq
-# refinement 7, LJ and Coulomb 10, the r^2 == 0 guard 3, the sum 1, two
+# object: 27 full-rate VALU instructions -- displacement 3, r^2 3, the rsq
+# refinement 5, LJ and Coulomb 10, the r^2 == 0 guard 3, the sum 1, two
 # address steps -- plus v_rsq_f64 at quarter rate = 4 slots); with 256 CUs
 # x 4 SIMDs x 16 lanes per cycle at 2.4 GHz this sets the pair-interaction
 # roof.
-VALU_PER_PAIR_FP64 = 33
+VALU_PER_PAIR_FP64 = 31
 VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
 
 

@@ -184,6 +184,12 @@ int mcgpu_last_kernel_ms(mcgpu_box* box, double* ms);
 int mcgpu_debug_math(const double* x, int32_t n, double* sin_out,
                      double* cos_out, double* exp_out);
 
+/* Test hook (no counterpart in the reference): evaluates the device FP64
+ * 1/sqrt(x) of the pair energies, which stands for the reference's
+ * r = sqrt(r2) and its divisions by r2 and r (SimulationStep.cpp:120,
+ * 344-367). */
+int mcgpu_debug_rsqrt(const double* x, int32_t n, double* out);
+
 /* ------------------------------------------------------------------ */
 /* Batched boxes: many independent boxes advanced by one persistent launch.
  * Each box carries its own device RNG (a glibc TYPE_3 stream seeded like

@@ -281,6 +281,12 @@ __global__ void k_debug_math(const double* x, int n, double* s, double* c,
   }
 }
 
+// Test hook: the FP64 1/sqrt of the pair energies.
+__global__ void k_debug_rsqrt(const double* x, int n, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = rsqrt64(x[i]);
+}
+
 // ------------------------------------------------------ launch helpers
 
 int launch_refresh_primary(const BoxView& bv, int n_pidx, hipStream_t s) {
@@ -1154,6 +1160,27 @@ extern "C" int mcgpu_debug_math(const double* x, int32_t n, double* s, double* c
   (void)hipFree(d);
   if (err != hipSuccess) {
     set_error(std::string("mcgpu_debug_math: ") + hipGetErrorString(err));
+    return MCGPU_ERR_HIP;
+  }
+  return MCGPU_OK;
+}
+
+extern "C" int mcgpu_debug_rsqrt(const double* x, int32_t n, double* y) {
+  if (!x || !y || n < 0) return MCGPU_ERR_INVALID;
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (n == 0) return MCGPU_OK;
+  double* d = nullptr;
+  MCGPU_HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+  hipError_t err = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_debug_rsqrt, dim3((n + 255) / 256), dim3(256), 0, 0, d, n, d + n);
+    err = hipDeviceSynchronize();
+  }
+  if (err == hipSuccess) err = hipMemcpy(y, d + n, sizeof(double) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) {
+    set_error(std::string("mcgpu_debug_rsqrt: ") + hipGetErrorString(err));
     return MCGPU_ERR_HIP;
   }
   return MCGPU_OK;

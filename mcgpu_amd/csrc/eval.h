@@ -25,6 +25,10 @@ namespace mcgpu {
 // Marks a staged site (in EvalShared::sp) of a hit whose site pairs may
 // wrap differently: its pairs take wrap1 per axis.
 constexpr int kPairWrap = 1 << 20;
+// A staged site's hit is in range of the moving molecule's old (kPairOld)
+// or proposed (kPairNew) position: its pairs enter e_old or e_new.
+constexpr int kPairOld = 1 << 21;
+constexpr int kPairNew = 1 << 22;
 
 template <int NT>
 struct EvalShared {
@@ -41,7 +45,8 @@ struct EvalShared {
   // site-parameter entry (-1: no such site in that hit); coordinates are
   // shifted by the periodic image of their pairs with the moving molecule,
   // except for hits whose site pairs may wrap differently (kPairWrap in sp
-  // marks those).
+  // marks those); sp also carries the hit's in-range flags (kPairOld,
+  // kPairNew).
   double sx[kSiteCap], sy[kSiteCap], sz[kSiteCap];
   int32_t sp[kSiteCap];
   // the box's site-parameter table
@@ -191,7 +196,7 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
         sh.sx[u] = bv.x[site] + (mixed ? 0.0 : image_shift(img & 3, L0));
         sh.sy[u] = bv.y[site] + (mixed ? 0.0 : image_shift((img >> 2) & 3, L1));
         sh.sz[u] = bv.z[site] + (mixed ? 0.0 : image_shift((img >> 4) & 3, L2));
-        pi = (sh.cand_param[k] + b) | (mixed ? kPairWrap : 0);
+        pi = (sh.cand_param[k] + b) | (mixed ? kPairWrap : 0) | ((lf >> 8) & 3) * kPairOld;
       }
       sh.sp[u] = pi;
     }
@@ -200,16 +205,14 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
     // their 16 waves instead and need the registers
     constexpr int U = NT >= 1024 ? 1 : 2;
     for (int i0 = threadIdx.x >> lg; i0 < tot; i0 += U * G) {
-      int pi[U], f[U];
+      int pi[U];
       double xb[U], yb[U], zb[U];
 #pragma unroll
       for (int v = 0; v < U; v++) {
         const int i = i0 + v * G;
         pi[v] = -1;
-        f[v] = 0;
         if (i < tot) {
           pi[v] = sh.sp[i];
-          f[v] = sh.cand_lf[h0 + div_small(i, magic)] >> 8;
           xb[v] = sh.sx[i];
           yb[v] = sh.sy[i];
           zb[v] = sh.sz[i];
@@ -236,21 +239,21 @@ __device__ __forceinline__ void pair_phase(const BoxView& bv, EvalShared<NT>& sh
         if (MIXED) {
           const SiteParam32 pb = sh.prm32[pk];
           const SiteParam64& pd = sh.prm64[pk];
-          if (f[v] & 1) {
+          if (pi[v] & kPairOld) {
             e_old += pair_energy32(dxo, dyo, dzo, sa32, ea32, pb.s, pb.e, sa, ea, qa, pd.s,
                                    pd.e, pd.q);
             npairs++;
           }
-          if (f[v] & 2)
+          if (pi[v] & kPairNew)
             e_new += pair_energy32(dxn, dyn, dzn, sa32, ea32, pb.s, pb.e, sa, ea, qa, pd.s,
                                    pd.e, pd.q);
         } else {
           const SiteParam64 pb = sh.prm64[pk];
-          if (f[v] & 1) {
+          if (pi[v] & kPairOld) {
             e_old += pair_energy64(dxo, dyo, dzo, sa, ea, qa, pb.s, pb.e, pb.q);
             npairs++;
           }
-          if (f[v] & 2)
+          if (pi[v] & kPairNew)
             e_new += pair_energy64(dxn, dyn, dzn, sa, ea, qa, pb.s, pb.e, pb.q);
         }
       }
