@@ -180,9 +180,10 @@ __device__ __forceinline__ double image_shift(int c, double L) {
 // 1/sqrt(x) in FP64: the hardware estimate y (v_rsq_f64, relative error
 // up to ~2^-22) refined by one third-order step of the binomial series
 // 1/sqrt(x) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...), e = 1 - x y^2
-// computed by an FMA.  The dropped terms are below 2^-64; the rounding of e
-// and of the final FMA keeps the result within ~1.5 ulp of the correctly
-// rounded value, in five FP64 operations (two Newton steps take seven).
+// computed by an FMA.  The dropped terms are below 2^-64; with every
+// operation rounded once, start errors up to 2^-22 leave the result within
+// 0.95 ulp of 1/sqrt(x) in an exact rational simulation of these steps, in
+// five FP64 operations (two Newton steps take seven).
 __device__ __forceinline__ double rsqrt64(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double e = fma(-(x * y), y, 1.0);
