@@ -121,10 +121,21 @@ class BoxPool:
 
 
 def neighbor_for(box) -> int:
-    """Cell lists once every dimension holds >= 3 cells of width rc."""
+    """Cell lists when they shrink the scan, else the all-pairs scan (both
+    give the same moves and energies): cells once every dimension holds at
+    least 4 cells of width rc, so that the 27 cells around a molecule cover
+    at most 27/64 of the box, and the box holds at least 1,000 molecules.
+    With 3 cells per edge the 27 cells are the whole box and the cell
+    bookkeeping only adds time.  Measured per move (FP64, one box alone, its
+    tuned work-group size): water of 692-1,303 molecules (3 cells) 7.7-9.4 us
+    all-pairs against 8.7-10.7 us with cells, bicyclic of 298-560 (3 cells)
+    13.3-14.3 against 14.4-15.4 us; with 4 cells, water of 1,609 molecules
+    9.4 against 8.8 us and of 2,454 11.1 against 9.6 us, bicyclic of 692 13.3
+    against 14.2 us and of 1,055 15.6 against 15.3 us."""
     import mcgpu_amd as m
     d = box.desc()
-    return m.NEIGHBOR_CELL if min(d.box) >= 3 * d.cutoff else m.NEIGHBOR_BRUTE
+    cells = min(d.box) >= 4 * d.cutoff and d.n_molecules >= 1000
+    return m.NEIGHBOR_CELL if cells else m.NEIGHBOR_BRUTE
 
 
 def run_kwargs(job, box):
