@@ -624,6 +624,8 @@ def main() -> int:
             "jobs": len(timed), "jobs_per_step": JOBS_PER_STEP,
             "first_job": n_warm, "moves": moves, "max_sweeps_per_tick": args.sweeps,
             "tick_budget_us": args.tick_us,
+            "neighbor": ("cell lists at >= 4 cells of width rc per edge and >= 1000 "
+                         "molecules, else all pairs (the plain path alike)"),
             "parallelism": "boxes in one persistent launch",
         },
         "p50_latency_s": float(np.median(lat)),
